@@ -1,0 +1,186 @@
+/*
+ * ipuma_hip.h -- C ABI of the MI355X batched Smith-Waterman (affine gap) engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes, no
+ * exceptions across the boundary.  Every entry point returns IPUMA_OK (0) or
+ * a negative IPUMA_E* code; ipuma_last_error() gives the message of the last
+ * failure on the calling thread.
+ *
+ * Each entry point cites the reference interface it replaces
+ * (xiamaz/ipuma-lib, file:line).  Results use the reference's packing
+ * (src/codelets/codelet_batch_affine.cpp:117-123):
+ *     a_range[k] = (uint16 Aend << 16) | uint16 Astart
+ *     b_range[k] = (uint16 Bend << 16) | uint16 Bstart
+ * With forward_only the starts are 0, as in the reference.
+ */
+#ifndef IPUMA_HIP_H
+#define IPUMA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPUMA_ABI_VERSION 1
+
+enum {
+  IPUMA_OK = 0,
+  IPUMA_EINVAL = -1,    /* bad argument (size, pointer, enum, alignment) */
+  IPUMA_ENOMEM = -2,    /* device or host allocation failed / over budget */
+  IPUMA_EDEVICE = -3,   /* HIP runtime or kernel failure */
+  IPUMA_ECONFIG = -4,   /* configuration the reference rejects (e.g. `simple`) */
+  IPUMA_ETOOLONG = -5,  /* sequence longer than the engine supports */
+  IPUMA_EBUSY = -6,     /* admission refused (request server) */
+  IPUMA_ECLOSED = -7    /* request server shut down */
+};
+
+/* swatlib::Similarity (src/swatlib/similarity.h:108-113). */
+enum { IPUMA_SIM_NA = 0, IPUMA_SIM_BLOSUM50 = 1, IPUMA_SIM_BLOSUM62 = 2, IPUMA_SIM_SIMPLE = 3 };
+/* swatlib::DataType (src/swatlib/encoding.h:12-16). */
+enum { IPUMA_DT_NA = 0, IPUMA_DT_AA = 1, IPUMA_DT_STRING = 2 };
+/* ipu::VertexType (src/types.h:19) and ipu::Algorithm (src/types.h:21). */
+enum { IPUMA_VT_CPP = 0, IPUMA_VT_ASSEMBLY = 1, IPUMA_VT_MULTI = 2, IPUMA_VT_MULTIASM = 3 };
+enum { IPUMA_FILL_FIRST = 0, IPUMA_ROUND_ROBIN = 1, IPUMA_GREEDY = 2 };
+
+/* Mirrors ipu::SWConfig field for field (src/ipu_config.h:11-19). */
+typedef struct ipuma_sw_config {
+  int32_t gap_init;        /* gapInit   (default 0)  */
+  int32_t gap_extend;      /* gapExtend (default -1) */
+  int32_t match_value;     /* matchValue (1)         */
+  int32_t mismatch_value;  /* mismatchValue (-1)     */
+  int32_t ambiguity_value; /* ambiguityValue (-1)    */
+  int32_t similarity;      /* IPUMA_SIM_*            */
+  int32_t datatype;        /* IPUMA_DT_*             */
+} ipuma_sw_config;
+
+/* Mirrors ipu::IPUAlgoConfig (src/ipu_config.h:21-31).  On MI355X the tile
+ * fields are capacity hints only: tiles_used*max_batches bounds the number of
+ * comparisons of one compare_local call (ipu_batch_affine.cpp:410-416) and
+ * max_ab bounds |A| (:418-423).  vtype / fill_algo / use_remote_buffer /
+ * transmission_programs / io_tiles are accepted and ignored. */
+typedef struct ipuma_algo_config {
+  int32_t tiles_used;
+  int32_t max_ab;
+  int32_t max_batches;
+  int32_t bufsize;
+  int32_t vtype;
+  int32_t fill_algo;
+  int32_t forward_only;
+  int32_t use_remote_buffer;
+  int32_t transmission_programs;
+  int32_t io_tiles;
+} ipuma_algo_config;
+
+typedef struct ipuma_engine ipuma_engine;
+
+/* Fill the reference defaults (src/ipu_config.h:11-31). */
+void ipuma_default_sw_config(ipuma_sw_config *c);
+void ipuma_default_algo_config(ipuma_algo_config *c);
+
+/* Replaces SWAlgorithm(SWConfig, IPUAlgoConfig[, thread_id, slotCap, ipuCount,
+ * runExecutor]) (src/ipu_batch_affine.h:76-77, .cpp:371-403): binds the
+ * current HIP device, builds the scoring table, creates the engine stream.
+ * `simple` similarity fails with IPUMA_ECONFIG like selectMatrix does
+ * (src/swatlib/similarity.cpp:100-101). */
+int ipuma_engine_create(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
+                        ipuma_engine **out);
+/* Replaces ~SWAlgorithm (src/ipu_batch_affine.cpp:1014-1022). */
+void ipuma_engine_destroy(ipuma_engine *e);
+
+/* Last error message of the calling thread ("" if none). */
+const char *ipuma_last_error(void);
+
+/*
+ * Replaces compare_local(A, B) + get_result()
+ * (src/ipu_batch_affine.h:96,99; .cpp:564-663, :408).
+ * Comparison k aligns A_k = a_chars[a_offsets[k] .. a_offsets[k+1]) against
+ * B_k = b_chars[b_offsets[k] .. b_offsets[k+1]).  Host buffers in and out;
+ * results are written for k < n.  Errors like the reference: more than
+ * tiles_used*max_batches comparisons or |A_k| > max_ab -> IPUMA_EINVAL /
+ * IPUMA_ETOOLONG (the reference throws / exits).
+ */
+int ipuma_compare_local(ipuma_engine *e, int32_t n,
+                        const char *a_chars, const int64_t *a_offsets,
+                        const char *b_chars, const int64_t *b_offsets,
+                        int32_t *scores, int32_t *a_range, int32_t *b_range);
+
+/*
+ * Replaces compare_mn_local(Seqs, Cmps) + get_result()
+ * (src/ipu_batch_affine.h:100; .cpp:697-750): comparison k aligns
+ * Seqs[cmps[2k]] (A) against Seqs[cmps[2k+1]] (B); ipu::Comparison is the
+ * packed {int32 indexA, int32 indexB} pair (src/types.h:12-15).  Each distinct
+ * sequence is encoded and uploaded once.
+ */
+int ipuma_compare_mn_local(ipuma_engine *e, int32_t nseqs, const char *chars,
+                           const int64_t *offsets, int32_t ncmps, const int32_t *cmps,
+                           int32_t *scores, int32_t *a_range, int32_t *b_range);
+
+/*
+ * Device-resident batch: the hot path with inputs already in HBM (what the
+ * device loop of src/ipu_batch_affine.cpp:340-365 consumes).  `codes` holds
+ * encoded residues (ipuma_encode_device or the host encoder); pair k aligns
+ * codes[a_off[k] .. +a_len[k]) against codes[b_off[k] .. +b_len[k]).  Offsets
+ * must be multiples of 4, every sequence must lie inside [0, codes_len) and
+ * `codes` must be readable up to the next multiple of 4 past every sequence.
+ * All pointers are device pointers; the call enqueues on `stream` (a
+ * hipStream_t, NULL = the engine's stream) and returns without waiting;
+ * ipuma_synchronize reports a length error (> 65535, positions are uint16).
+ * Codes outside the matrix are not allowed (ipuma_encode_device never makes
+ * them).
+ */
+int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
+                         const int64_t *a_off, const int32_t *a_len,
+                         const int64_t *b_off, const int32_t *b_len,
+                         int32_t *scores, int32_t *a_range, int32_t *b_range,
+                         void *stream);
+
+/* Batching policy of the device path (replaces IPUAlgoConfig.fillAlgo's
+ * partition::fillFirst/roundRobin/greedy, src/partition.cpp:197-462). */
+enum {
+  IPUMA_BATCH_BUCKETED = 0, /* device length bucketing + persistent queue (default) */
+  IPUMA_BATCH_PLAIN = 1     /* one pair per lane, natural order, padded to batch max */
+};
+/* Precision policy of the device path. */
+enum {
+  IPUMA_PREC_AUTO = 0,  /* packed int16, exact int32 re-run of saturated pairs */
+  IPUMA_PREC_INT32 = 1  /* int32 only */
+};
+int ipuma_set_batching(ipuma_engine *e, int32_t mode);
+int ipuma_set_precision(ipuma_engine *e, int32_t mode);
+int ipuma_set_forward_only(ipuma_engine *e, int32_t forward_only);
+/* Lanes per comparison of the packed kernel: 0 = chosen per comparison by the
+ * bucketing (default), or one of 1, 2, 4, 8, 16, 64 for every comparison.
+ * No reference counterpart (the IPU runs one comparison per worker). */
+int ipuma_set_lanes_per_pair(ipuma_engine *e, int32_t lanes);
+
+/* Encode characters to residue codes on the device with the engine's
+ * encoding (swatlib Encoding, src/swatlib/encoding.cpp:40-62). */
+int ipuma_encode_device(ipuma_engine *e, const char *d_chars, int64_t len,
+                        uint8_t *d_codes, void *stream);
+
+/* Statistics of the last device batch (for logging / the bench). */
+typedef struct ipuma_batch_stats {
+  int64_t pairs;           /* comparisons */
+  int64_t cells;           /* sum |A|*|B| */
+  int64_t padded_cells;    /* cells the kernels computed, padding included */
+  int64_t groups;          /* work groups of the persistent kernel */
+  int64_t rerun_pairs;     /* pairs re-run in int32 after int16 saturation */
+  int32_t lanes_per_pair_max;
+  int32_t rows_per_lane;
+  double fast_kernel_ms;   /* packed kernel, HIP events on its stream (timing on) */
+  double batch_ms;         /* whole device batch, same events (timing on) */
+} ipuma_batch_stats;
+/* Record HIP events around the packed kernel and the whole device batch of the
+ * following calls (read back by ipuma_last_batch_stats). */
+int ipuma_set_timing(ipuma_engine *e, int32_t on);
+int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out);
+
+/* Synchronize the engine stream. */
+int ipuma_synchronize(ipuma_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IPUMA_HIP_H */

@@ -1,0 +1,140 @@
+// sw_device.h -- shared host/device definitions of the MI355X Smith-Waterman engine.
+#pragma once
+#include <stdint.h>
+
+namespace ipuma {
+
+// Rows of one lane's strip held in registers by the packed kernel.
+constexpr int kRows = 16;
+// Threads per block of the packed kernel (one block per CU: the replicated
+// scoring table takes most of the LDS).
+constexpr int kFastThreads = 512;
+// Lanes-per-pair classes of the packed kernel (pipeline depth inside a wave).
+constexpr int kNumClasses = 6;
+constexpr int kClassL[kNumClasses] = {1, 2, 4, 8, 16, 64};
+// Boundary scratch of one wave, in 32-bit words: mpad * (64/L) * 2 <= this.
+// The class rules keep mpad * (64/L) <= 65536, so 512 KiB per wave.
+constexpr int64_t kWaveScratchWords = 65536 * 2;
+// Longest sequence the engine accepts (positions are packed as uint16).
+constexpr int32_t kMaxSeqLen = 65535;
+// Counting-sort bins of the length bucketing: class x super-strips x m-bin.
+constexpr int kNssBins = 64;
+constexpr int kMBins = 256;
+constexpr int kMBinWidth = 16;
+constexpr int kNumBins = kNumClasses * kNssBins * kMBins;
+// Group-cost bins for the longest-first queue order.
+constexpr int kCostBins = 512;
+// Saturation value of the packed int16 path.
+constexpr int32_t kSat16 = 32767;
+
+// Group descriptor of the persistent queue.
+struct GroupDesc {
+  int32_t cls;   // class index into kClassL
+  int32_t mpad;  // columns (|A|) padded for the group
+  int32_t nss;   // super-strips (rows / (L * kRows), rounded up)
+  int32_t base;  // first slot in the sorted pair array
+};
+
+// Control block of the device bucketing (one per batch, zeroed before use).
+struct BucketCtl {
+  int32_t class_count[kNumClasses];
+  int32_t class_start[kNumClasses];  // aligned start of the class in `sorted`
+  int32_t class_shift[kNumClasses];  // aligned start - unaligned prefix
+  int32_t group_base[kNumClasses + 1];
+  int32_t ngroups;
+  int32_t error;       // bit 0: a length outside [0, kMaxSeqLen]
+  int32_t ovf_count;   // int16-saturated comparisons queued for int32
+  int32_t queue;       // persistent-kernel group counter
+  int32_t maxm;        // plain batching: longest A
+  int32_t maxn;        // plain batching: longest B
+  int32_t pad0;
+  unsigned long long total_cells;
+  unsigned long long padded_cells;
+};
+
+// Arguments of the packed kernel.
+struct FastArgs {
+  const uint8_t *codes;
+  const int64_t *a_off;
+  const int32_t *a_len;
+  const int64_t *b_off;
+  const int32_t *b_len;
+  const int16_t *table;        // KP x KP, row = A code, col = B code
+  const int32_t *sorted;       // pair ids in bucket order (-1 = empty slot)
+  const GroupDesc *groups;     // in processing order
+  const int32_t *ngroups;      // device scalar
+  int32_t *queue;              // device counter (zeroed before launch)
+  uint32_t *scratch;           // kWaveScratchWords per wave
+  int32_t *scores;
+  int32_t *a_range;
+  int32_t *b_range;
+  int32_t *ovf_list;           // pairs that saturated int16
+  int32_t *ovf_count;
+  uint32_t gap_o;              // -(gapInit + gapExtend), packed twice
+  uint32_t gap_x;              // -gapExtend, packed twice
+};
+
+// Arguments of the exact int32 kernel.
+struct GenericArgs {
+  const uint8_t *codes;
+  const int64_t *a_off;
+  const int32_t *a_len;
+  const int64_t *b_off;
+  const int32_t *b_len;
+  const int16_t *table;
+  int32_t kp;
+  const int32_t *list;   // pair ids, or nullptr for 0..n-1
+  const int32_t *count;  // device count of `list`, or nullptr
+  int32_t n;             // pair count when list == nullptr
+  int32_t *work;         // 2 * 65536 ints per wave
+  int32_t *error;        // set to 1 when a length exceeds kMaxSeqLen
+  int32_t gap_init;
+  int32_t gap_extend;
+  int32_t *scores;
+  int32_t *a_range;
+  int32_t *b_range;
+};
+
+// Arguments of the bucketing pipeline.
+struct BucketArgs {
+  int32_t n;
+  const int32_t *a_len;
+  const int32_t *b_len;
+  int64_t target_slots;   // resident half-lanes: fair share = cells / this
+  int32_t force_class;    // -1: choose per pair
+  int32_t plain;          // 1: input order, L = 1, batch-max padding
+  int32_t max_groups;
+  int32_t *bin_of;
+  int32_t *hist;          // kNumBins
+  int32_t *costhist;      // kCostBins
+  BucketCtl *ctl;
+  int32_t *sorted;        // n + 1024, pre-filled with -1
+  GroupDesc *gtmp;
+  int32_t *gbin;
+  GroupDesc *groups;
+  int32_t *scores;
+  int32_t *a_range;
+  int32_t *b_range;
+};
+
+}  // namespace ipuma
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+namespace ipuma {
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+hipError_t launch_encode(const uint8_t *chars, int64_t len, uint8_t *codes, const uint8_t *table,
+                         hipStream_t st);
+hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st);
+hipError_t launch_fast(int kp, const FastArgs &A, int blocks, hipStream_t st);
+int fast_blocks_per_cu(int kp);
+hipError_t launch_generic(const GenericArgs &G, int blocks, hipStream_t st);
+hipError_t launch_reverse_prefix(int32_t n, const uint8_t *codes, const int64_t *a_off,
+                                 const int64_t *b_off, const int32_t *scores,
+                                 const int32_t *a_range, const int32_t *b_range, uint8_t *rcodes,
+                                 int32_t *ra_len, int32_t *rb_len, hipStream_t st);
+hipError_t launch_finish_starts(int32_t n, const int32_t *scores, const int32_t *ra_range,
+                                const int32_t *rb_range, int32_t *a_range, int32_t *b_range,
+                                hipStream_t st);
+}  // namespace ipuma
+#endif

@@ -1,0 +1,585 @@
+// sw_engine.cpp -- host side of the MI355X Smith-Waterman engine and its C ABI.
+//
+// One engine = one HIP device + one stream + the scoring table in HBM + grow-
+// only scratch.  A device batch is a fixed chain of kernels on one stream,
+// with no host round trip:
+//   bucketing (k_sum_cells, k_classify, k_scan_bins, k_scatter, k_groups,
+//   k_scan_cost, k_order_groups)  ->  k_fast (packed int16, persistent)  ->
+//   k_generic32 (exact int32 re-run of saturated comparisons)  ->
+//   [k_reverse_prefix -> same chain on the reversed prefixes -> k_finish_starts]
+// The last block only runs without forward_only, for start positions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ipuma_hip.h"
+#include "sw_device.h"
+#include "sw_tables.h"
+
+using namespace ipuma;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(IPUMA_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                  \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return IPUMA_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&p, want) != hipSuccess) {
+      p = nullptr;
+      return fail(IPUMA_ENOMEM, "hipMalloc(%zu) failed", want);
+    }
+    cap = want;
+    return IPUMA_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+// Device state of one bucketing + kernel chain (the forward pass and the
+// reverse pass each own one, so they can be in flight together).
+struct ChainBufs {
+  DevBuf bin_of, hist, costhist, ctl, sorted, gtmp, gbin, groups, ovf;
+  void release() {
+    for (DevBuf *b : {&bin_of, &hist, &costhist, &ctl, &sorted, &gtmp, &gbin, &groups, &ovf})
+      b->release();
+  }
+};
+
+}  // namespace
+
+struct ipuma_engine {
+  ipuma_sw_config sw{};
+  ipuma_algo_config algo{};
+  int device = 0;
+  int num_cu = 0;
+  int kp = 0;   // padded alphabet (matrix side + pad code)
+  int dim = 0;  // matrix side
+  hipStream_t stream = nullptr;
+  std::vector<int16_t> h_table;
+  uint8_t h_codes[256];
+  int16_t *d_table = nullptr;
+  uint8_t *d_codetab = nullptr;
+  int batching = IPUMA_BATCH_BUCKETED;
+  int precision = IPUMA_PREC_AUTO;
+  int forward_only = 0;
+  int force_class = -1;
+  bool packed_ok = false;  // gap settings inside the packed kernel's domain
+  int fast_blocks = 0;
+  int generic_blocks = 512;
+  int64_t target_slots = 0;
+  DevBuf scratch, work;
+  ChainBufs fwd, rev;
+  DevBuf rcodes, rlen, rres;
+  // host staging of compare_local / compare_mn_local
+  DevBuf in_chars, in_codes, in_meta, out_res;
+  std::vector<char> h_chars;
+  std::vector<int64_t> h_aoff, h_boff;
+  std::vector<int32_t> h_alen, h_blen;
+  ipuma_batch_stats stats{};
+  BucketCtl *last_ctl = nullptr;  // device ctl of the last forward chain
+  int32_t last_n = 0;
+  int timing = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // batch begin, fast begin/end, batch end
+  bool ev_recorded = false;
+};
+
+namespace {
+
+int init_engine(ipuma_engine *e) {
+  HIP_TRY(hipGetDevice(&e->device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, e->device));
+  e->num_cu = prop.multiProcessorCount;
+  HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  std::vector<int8_t> mat(25 * 25);
+  const int dim = select_matrix(e->sw.similarity, e->sw.match_value, e->sw.mismatch_value,
+                                e->sw.ambiguity_value, mat.data());
+  if (dim < 0) return fail(IPUMA_ECONFIG, "No matrix for simple penalty available.");
+  e->dim = dim;
+  e->kp = dim + 1;
+  e->h_table.resize((size_t)e->kp * e->kp);
+  build_pad_table(mat.data(), dim, e->h_table.data());
+  build_code_table(e->sw.datatype, e->h_codes);
+  // codes outside the matrix (e.g. a nucleic-acid encoding with an amino-acid
+  // matrix) would index past the table: clamp them to the last real code.
+  for (int i = 0; i < 256; ++i)
+    if (e->h_codes[i] >= dim) e->h_codes[i] = (uint8_t)(dim - 1);
+  HIP_TRY(hipMalloc(&e->d_table, e->h_table.size() * sizeof(int16_t)));
+  HIP_TRY(hipMemcpy(e->d_table, e->h_table.data(), e->h_table.size() * sizeof(int16_t),
+                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&e->d_codetab, 256));
+  HIP_TRY(hipMemcpy(e->d_codetab, e->h_codes, 256, hipMemcpyHostToDevice));
+  const int o = -(e->sw.gap_init + e->sw.gap_extend), x = -e->sw.gap_extend;
+  e->packed_ok = o >= 0 && x >= 0 && o <= 32767 && x <= 32767;
+  e->forward_only = e->algo.forward_only ? 1 : 0;
+  e->fast_blocks = e->num_cu * fast_blocks_per_cu(e->kp);
+  e->target_slots = (int64_t)e->fast_blocks * (kFastThreads / 64) * 128;
+  return IPUMA_OK;
+}
+
+int ensure_chain(ChainBufs &c, int32_t n) {
+  int rc;
+  const size_t nn = (size_t)std::max(n, 1);
+  const size_t max_groups = nn + 64;
+  if ((rc = c.bin_of.ensure(nn * 4)) || (rc = c.hist.ensure((size_t)kNumBins * 4)) ||
+      (rc = c.costhist.ensure((size_t)kCostBins * 4)) || (rc = c.ctl.ensure(sizeof(BucketCtl))) ||
+      (rc = c.sorted.ensure((nn + 1024) * 4)) || (rc = c.gtmp.ensure(max_groups * sizeof(GroupDesc))) ||
+      (rc = c.gbin.ensure(max_groups * 4)) || (rc = c.groups.ensure(max_groups * sizeof(GroupDesc))) ||
+      (rc = c.ovf.ensure(nn * 4)))
+    return rc;
+  return IPUMA_OK;
+}
+
+// One forward chain: bucketing -> packed kernel -> int32 re-run.
+int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
+              const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
+              const int32_t *b_len, int32_t *scores, int32_t *a_range, int32_t *b_range,
+              hipStream_t st) {
+  int rc = ensure_chain(c, n);
+  if (rc) return rc;
+  const bool packed = e->packed_ok && e->precision == IPUMA_PREC_AUTO;
+  BucketCtl *ctl = c.ctl.as<BucketCtl>();
+  HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(ctl), sizeof(BucketCtl) / 4, 0, st));
+  if (!packed) {
+    // every comparison through the exact int32 kernel
+    if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
+    GenericArgs G{};
+    G.codes = codes; G.a_off = a_off; G.a_len = a_len; G.b_off = b_off; G.b_len = b_len;
+    G.table = e->d_table; G.kp = e->kp; G.list = nullptr; G.count = nullptr; G.n = n;
+    G.work = e->work.as<int32_t>(); G.gap_init = e->sw.gap_init; G.gap_extend = e->sw.gap_extend;
+    G.scores = scores; G.a_range = a_range; G.b_range = b_range; G.error = &ctl->error;
+    HIP_TRY(launch_generic(G, e->generic_blocks, st));
+    return IPUMA_OK;
+  }
+  const size_t scratch_bytes = (size_t)e->fast_blocks * (kFastThreads / 64) * kWaveScratchWords * 4;
+  if ((rc = e->scratch.ensure(scratch_bytes))) return rc;
+  if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
+  HIP_TRY(launch_fill_i32(c.hist.as<int32_t>(), kNumBins, 0, st));
+  HIP_TRY(launch_fill_i32(c.costhist.as<int32_t>(), kCostBins, 0, st));
+  HIP_TRY(launch_fill_i32(c.sorted.as<int32_t>(), (int64_t)n + 1024, -1, st));
+  BucketArgs B{};
+  B.n = n; B.a_len = a_len; B.b_len = b_len; B.target_slots = e->target_slots;
+  B.force_class = e->force_class; B.plain = e->batching == IPUMA_BATCH_PLAIN ? 1 : 0;
+  B.max_groups = n + 64; B.bin_of = c.bin_of.as<int32_t>(); B.hist = c.hist.as<int32_t>();
+  B.costhist = c.costhist.as<int32_t>(); B.ctl = ctl; B.sorted = c.sorted.as<int32_t>();
+  B.gtmp = c.gtmp.as<GroupDesc>(); B.gbin = c.gbin.as<int32_t>(); B.groups = c.groups.as<GroupDesc>();
+  B.scores = scores; B.a_range = a_range; B.b_range = b_range;
+  HIP_TRY(launch_bucketing(B, st));
+  FastArgs A{};
+  A.codes = codes; A.a_off = a_off; A.a_len = a_len; A.b_off = b_off; A.b_len = b_len;
+  A.table = e->d_table; A.sorted = c.sorted.as<int32_t>(); A.groups = c.groups.as<GroupDesc>();
+  A.ngroups = &ctl->ngroups; A.queue = &ctl->queue; A.scratch = e->scratch.as<uint32_t>();
+  A.scores = scores; A.a_range = a_range; A.b_range = b_range;
+  A.ovf_list = c.ovf.as<int32_t>(); A.ovf_count = &ctl->ovf_count;
+  const uint32_t o = (uint32_t)(-(e->sw.gap_init + e->sw.gap_extend)) & 0xFFFFu;
+  const uint32_t x = (uint32_t)(-e->sw.gap_extend) & 0xFFFFu;
+  A.gap_o = o | (o << 16);
+  A.gap_x = x | (x << 16);
+  const bool time_fast = e->timing && &c == &e->fwd;
+  if (time_fast) HIP_TRY(hipEventRecord(e->ev[1], st));
+  HIP_TRY(launch_fast(e->kp, A, e->fast_blocks, st));
+  if (time_fast) HIP_TRY(hipEventRecord(e->ev[2], st));
+  GenericArgs G{};
+  G.codes = codes; G.a_off = a_off; G.a_len = a_len; G.b_off = b_off; G.b_len = b_len;
+  G.table = e->d_table; G.kp = e->kp; G.list = c.ovf.as<int32_t>(); G.count = &ctl->ovf_count;
+  G.n = 0; G.work = e->work.as<int32_t>(); G.gap_init = e->sw.gap_init;
+  G.gap_extend = e->sw.gap_extend; G.scores = scores; G.a_range = a_range; G.b_range = b_range;
+  G.error = &ctl->error;
+  HIP_TRY(launch_generic(G, e->generic_blocks, st));
+  return IPUMA_OK;
+}
+
+int compare_device_impl(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
+                        const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
+                        const int32_t *b_len, int32_t *scores, int32_t *a_range,
+                        int32_t *b_range, hipStream_t st) {
+  if (e->timing) HIP_TRY(hipEventRecord(e->ev[0], st));
+  e->ev_recorded = false;
+  int rc = run_chain(e, e->fwd, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st);
+  if (rc) return rc;
+  e->last_ctl = e->fwd.ctl.as<BucketCtl>();
+  e->last_n = n;
+  if (e->forward_only || n == 0) {
+    if (e->timing) {
+      HIP_TRY(hipEventRecord(e->ev[3], st));
+      e->ev_recorded = e->packed_ok && e->precision == IPUMA_PREC_AUTO;
+    }
+    return IPUMA_OK;
+  }
+  // start positions: forward pass over the reversed prefixes
+  if ((rc = e->rcodes.ensure((size_t)codes_len + 16)) || (rc = e->rlen.ensure((size_t)n * 8)) ||
+      (rc = e->rres.ensure((size_t)n * 12)))
+    return rc;
+  int32_t *ra_len = e->rlen.as<int32_t>(), *rb_len = ra_len + n;
+  int32_t *rs = e->rres.as<int32_t>(), *rar = rs + n, *rbr = rs + 2 * (size_t)n;
+  HIP_TRY(launch_reverse_prefix(n, codes, a_off, b_off, scores, a_range, b_range,
+                                e->rcodes.as<uint8_t>(), ra_len, rb_len, st));
+  rc = run_chain(e, e->rev, n, e->rcodes.as<uint8_t>(), a_off, ra_len, b_off, rb_len, rs, rar, rbr,
+                 st);
+  if (rc) return rc;
+  HIP_TRY(launch_finish_starts(n, scores, rar, rbr, a_range, b_range, st));
+  if (e->timing) {
+    HIP_TRY(hipEventRecord(e->ev[3], st));
+    e->ev_recorded = e->packed_ok && e->precision == IPUMA_PREC_AUTO;
+  }
+  return IPUMA_OK;
+}
+
+// Host strings -> 4-aligned device arena of codes (encoded on the device).
+int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
+                const std::vector<int64_t> &lens, std::vector<int64_t> &offs, int64_t &total) {
+  offs.resize(ptrs.size());
+  int64_t pos = 0;
+  for (size_t i = 0; i < ptrs.size(); ++i) {
+    offs[i] = pos;
+    pos += (lens[i] + 3) / 4 * 4;
+  }
+  total = pos + 16;
+  e->h_chars.assign((size_t)total, 0);
+  for (size_t i = 0; i < ptrs.size(); ++i)
+    if (lens[i]) std::memcpy(e->h_chars.data() + offs[i], ptrs[i], (size_t)lens[i]);
+  int rc;
+  if ((rc = e->in_chars.ensure((size_t)total)) || (rc = e->in_codes.ensure((size_t)total))) return rc;
+  HIP_TRY(hipMemcpyAsync(e->in_chars.p, e->h_chars.data(), (size_t)total, hipMemcpyHostToDevice,
+                         e->stream));
+  HIP_TRY(launch_encode(e->in_chars.as<uint8_t>(), total, e->in_codes.as<uint8_t>(), e->d_codetab,
+                        e->stream));
+  return IPUMA_OK;
+}
+
+int run_host_batch(ipuma_engine *e, int32_t n, int64_t codes_len, int32_t *scores,
+                   int32_t *a_range, int32_t *b_range) {
+  int rc;
+  const size_t meta_bytes = (size_t)n * (8 + 4 + 8 + 4);
+  if ((rc = e->in_meta.ensure(meta_bytes)) || (rc = e->out_res.ensure((size_t)n * 12 + 16)))
+    return rc;
+  char *m = e->in_meta.as<char>();
+  int64_t *d_aoff = reinterpret_cast<int64_t *>(m);
+  int64_t *d_boff = d_aoff + n;
+  int32_t *d_alen = reinterpret_cast<int32_t *>(d_boff + n);
+  int32_t *d_blen = d_alen + n;
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(d_aoff, e->h_aoff.data(), (size_t)n * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(d_boff, e->h_boff.data(), (size_t)n * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(d_alen, e->h_alen.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(d_blen, e->h_blen.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream));
+  }
+  int32_t *d_s = e->out_res.as<int32_t>(), *d_ar = d_s + n, *d_br = d_s + 2 * (size_t)n;
+  rc = compare_device_impl(e, n, e->in_codes.as<uint8_t>(), codes_len, d_aoff, d_alen, d_boff,
+                           d_blen, d_s, d_ar, d_br, e->stream);
+  if (rc) return rc;
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(scores, d_s, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(a_range, d_ar, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(b_range, d_br, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  int32_t err = 0;
+  HIP_TRY(hipMemcpy(&err, &e->fwd.ctl.as<BucketCtl>()->error, 4, hipMemcpyDeviceToHost));
+  if (err) return fail(IPUMA_ETOOLONG, "a sequence is longer than %d", kMaxSeqLen);
+  return IPUMA_OK;
+}
+
+int check_sizes(ipuma_engine *e, int32_t n, const int64_t *a_lens_or_null,
+                const std::vector<int64_t> &a_lens) {
+  (void)a_lens_or_null;
+  const int64_t cap = (int64_t)e->algo.tiles_used * e->algo.max_batches;
+  if (cap > 0 && n > cap)
+    return fail(IPUMA_EINVAL,
+                "Input sequence (A) is over the max length: %d comparisons > tilesUsed*maxBatches = %lld",
+                n, (long long)cap);
+  for (int64_t l : a_lens)
+    if (e->algo.max_ab > 0 && l > e->algo.max_ab)
+      return fail(IPUMA_ETOOLONG, "Sequence size in seq %lld > maxAB %d", (long long)l,
+                  e->algo.max_ab);
+  return IPUMA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ipuma_default_sw_config(ipuma_sw_config *c) {
+  if (!c) return;
+  c->gap_init = 0;
+  c->gap_extend = -1;
+  c->match_value = 1;
+  c->mismatch_value = -1;
+  c->ambiguity_value = -1;
+  c->similarity = IPUMA_SIM_NA;
+  c->datatype = IPUMA_DT_NA;
+}
+
+void ipuma_default_algo_config(ipuma_algo_config *c) {
+  if (!c) return;
+  c->tiles_used = 1;
+  c->max_ab = 300;
+  c->max_batches = 20;
+  c->bufsize = 3000;
+  c->vtype = IPUMA_VT_CPP;
+  c->fill_algo = IPUMA_FILL_FIRST;
+  c->forward_only = 0;
+  c->use_remote_buffer = 0;
+  c->transmission_programs = 1;
+  c->io_tiles = 0;
+}
+
+const char *ipuma_last_error(void) { return g_last_error.c_str(); }
+
+int ipuma_engine_create(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
+                        ipuma_engine **out) {
+  if (!out) return fail(IPUMA_EINVAL, "out is NULL");
+  *out = nullptr;
+  auto *e = new (std::nothrow) ipuma_engine();
+  if (!e) return fail(IPUMA_ENOMEM, "host allocation failed");
+  if (sw) e->sw = *sw; else ipuma_default_sw_config(&e->sw);
+  if (algo) e->algo = *algo; else ipuma_default_algo_config(&e->algo);
+  if (e->sw.similarity < 0 || e->sw.similarity > 3 || e->sw.datatype < 0 || e->sw.datatype > 2) {
+    delete e;
+    return fail(IPUMA_EINVAL, "invalid similarity/datatype");
+  }
+  const int rc = init_engine(e);
+  if (rc) {
+    ipuma_engine_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return IPUMA_OK;
+}
+
+void ipuma_engine_destroy(ipuma_engine *e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  e->scratch.release();
+  e->work.release();
+  e->fwd.release();
+  e->rev.release();
+  e->rcodes.release();
+  e->rlen.release();
+  e->rres.release();
+  e->in_chars.release();
+  e->in_codes.release();
+  e->in_meta.release();
+  e->out_res.release();
+  for (auto &ev : e->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->d_table) (void)hipFree(e->d_table);
+  if (e->d_codetab) (void)hipFree(e->d_codetab);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+int ipuma_compare_local(ipuma_engine *e, int32_t n, const char *a_chars, const int64_t *a_offsets,
+                        const char *b_chars, const int64_t *b_offsets, int32_t *scores,
+                        int32_t *a_range, int32_t *b_range) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  if (n < 0) return fail(IPUMA_EINVAL, "n < 0");
+  if (n > 0 && (!a_offsets || !b_offsets || !scores || !a_range || !b_range))
+    return fail(IPUMA_EINVAL, "NULL buffer");
+  std::vector<const char *> ptrs(2 * (size_t)n);
+  std::vector<int64_t> lens(2 * (size_t)n), alens((size_t)n);
+  for (int32_t k = 0; k < n; ++k) {
+    const int64_t la = a_offsets[k + 1] - a_offsets[k], lb = b_offsets[k + 1] - b_offsets[k];
+    if (la < 0 || lb < 0) return fail(IPUMA_EINVAL, "offsets not ascending at %d", k);
+    if (la > kMaxSeqLen || lb > kMaxSeqLen)
+      return fail(IPUMA_ETOOLONG, "sequence %d longer than %d", k, kMaxSeqLen);
+    ptrs[2 * k] = a_chars + a_offsets[k];
+    ptrs[2 * k + 1] = b_chars + b_offsets[k];
+    lens[2 * k] = la;
+    lens[2 * k + 1] = lb;
+    alens[k] = la;
+  }
+  int rc = check_sizes(e, n, nullptr, alens);
+  if (rc) return rc;
+  std::vector<int64_t> offs;
+  int64_t total = 0;
+  if ((rc = stage_arena(e, ptrs, lens, offs, total))) return rc;
+  e->h_aoff.resize(n); e->h_boff.resize(n); e->h_alen.resize(n); e->h_blen.resize(n);
+  for (int32_t k = 0; k < n; ++k) {
+    e->h_aoff[k] = offs[2 * k];
+    e->h_boff[k] = offs[2 * k + 1];
+    e->h_alen[k] = (int32_t)lens[2 * k];
+    e->h_blen[k] = (int32_t)lens[2 * k + 1];
+  }
+  return run_host_batch(e, n, total, scores, a_range, b_range);
+}
+
+int ipuma_compare_mn_local(ipuma_engine *e, int32_t nseqs, const char *chars,
+                           const int64_t *offsets, int32_t ncmps, const int32_t *cmps,
+                           int32_t *scores, int32_t *a_range, int32_t *b_range) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  if (nseqs < 0 || ncmps < 0) return fail(IPUMA_EINVAL, "negative count");
+  if (ncmps > 0 && (!cmps || !scores || !a_range || !b_range || !offsets))
+    return fail(IPUMA_EINVAL, "NULL buffer");
+  std::vector<const char *> ptrs((size_t)nseqs);
+  std::vector<int64_t> lens((size_t)nseqs);
+  for (int32_t s = 0; s < nseqs; ++s) {
+    lens[s] = offsets[s + 1] - offsets[s];
+    if (lens[s] < 0) return fail(IPUMA_EINVAL, "offsets not ascending at %d", s);
+    if (lens[s] > kMaxSeqLen) return fail(IPUMA_ETOOLONG, "sequence %d longer than %d", s, kMaxSeqLen);
+    ptrs[s] = chars + offsets[s];
+  }
+  std::vector<int64_t> alens((size_t)ncmps);
+  for (int32_t k = 0; k < ncmps; ++k) {
+    const int32_t ia = cmps[2 * k], ib = cmps[2 * k + 1];
+    if (ia < 0 || ia >= nseqs || ib < 0 || ib >= nseqs)
+      return fail(IPUMA_EINVAL, "comparison %d indexes outside the sequences", k);
+    alens[k] = lens[ia];
+  }
+  int rc = check_sizes(e, ncmps, nullptr, alens);
+  if (rc) return rc;
+  std::vector<int64_t> offs;
+  int64_t total = 0;
+  if ((rc = stage_arena(e, ptrs, lens, offs, total))) return rc;
+  e->h_aoff.resize(ncmps); e->h_boff.resize(ncmps); e->h_alen.resize(ncmps); e->h_blen.resize(ncmps);
+  for (int32_t k = 0; k < ncmps; ++k) {
+    const int32_t ia = cmps[2 * k], ib = cmps[2 * k + 1];
+    e->h_aoff[k] = offs[ia];
+    e->h_boff[k] = offs[ib];
+    e->h_alen[k] = (int32_t)lens[ia];
+    e->h_blen[k] = (int32_t)lens[ib];
+  }
+  return run_host_batch(e, ncmps, total, scores, a_range, b_range);
+}
+
+int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
+                         const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
+                         const int32_t *b_len, int32_t *scores, int32_t *a_range,
+                         int32_t *b_range, void *stream) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  if (n < 0) return fail(IPUMA_EINVAL, "n < 0");
+  if (n > 0 && (!codes || !a_off || !a_len || !b_off || !b_len || !scores || !a_range || !b_range))
+    return fail(IPUMA_EINVAL, "NULL buffer");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : e->stream;
+  return compare_device_impl(e, n, codes, codes_len, a_off, a_len, b_off, b_len, scores, a_range,
+                             b_range, st);
+}
+
+int ipuma_set_batching(ipuma_engine *e, int32_t mode) {
+  if (!e || (mode != IPUMA_BATCH_BUCKETED && mode != IPUMA_BATCH_PLAIN))
+    return fail(IPUMA_EINVAL, "invalid batching mode");
+  e->batching = mode;
+  return IPUMA_OK;
+}
+
+int ipuma_set_precision(ipuma_engine *e, int32_t mode) {
+  if (!e || (mode != IPUMA_PREC_AUTO && mode != IPUMA_PREC_INT32))
+    return fail(IPUMA_EINVAL, "invalid precision mode");
+  e->precision = mode;
+  return IPUMA_OK;
+}
+
+int ipuma_set_forward_only(ipuma_engine *e, int32_t forward_only) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  e->forward_only = forward_only ? 1 : 0;
+  return IPUMA_OK;
+}
+
+int ipuma_set_timing(ipuma_engine *e, int32_t on) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  if (on && !e->ev[0])
+    for (auto &ev : e->ev) HIP_TRY(hipEventCreate(&ev));
+  e->timing = on ? 1 : 0;
+  return IPUMA_OK;
+}
+
+int ipuma_set_lanes_per_pair(ipuma_engine *e, int32_t lanes) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  if (lanes == 0) {
+    e->force_class = -1;
+    return IPUMA_OK;
+  }
+  for (int c = 0; c < kNumClasses; ++c)
+    if (kClassL[c] == lanes) {
+      e->force_class = c;
+      return IPUMA_OK;
+    }
+  return fail(IPUMA_EINVAL, "lanes per pair must be 0, 1, 2, 4, 8, 16 or 64");
+}
+
+int ipuma_encode_device(ipuma_engine *e, const char *d_chars, int64_t len, uint8_t *d_codes,
+                        void *stream) {
+  if (!e || len < 0 || (len > 0 && (!d_chars || !d_codes))) return fail(IPUMA_EINVAL, "bad args");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : e->stream;
+  HIP_TRY(launch_encode(reinterpret_cast<const uint8_t *>(d_chars), len, d_codes, e->d_codetab, st));
+  return IPUMA_OK;
+}
+
+int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out) {
+  if (!e || !out) return fail(IPUMA_EINVAL, "bad args");
+  std::memset(out, 0, sizeof(*out));
+  if (!e->last_ctl) return IPUMA_OK;
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(hipDeviceSynchronize());
+  BucketCtl c;
+  HIP_TRY(hipMemcpy(&c, e->last_ctl, sizeof(c), hipMemcpyDeviceToHost));
+  out->pairs = e->last_n;
+  out->cells = (int64_t)c.total_cells;
+  out->padded_cells = (int64_t)c.padded_cells;
+  out->groups = c.ngroups;
+  out->rerun_pairs = c.ovf_count;
+  int lmax = 1;
+  for (int k = 0; k < kNumClasses; ++k)
+    if (c.class_count[k] > 0) lmax = kClassL[k];
+  out->lanes_per_pair_max = lmax;
+  out->rows_per_lane = kRows;
+  if (e->ev_recorded) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev[1], e->ev[2]));
+    out->fast_kernel_ms = ms;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev[0], e->ev[3]));
+    out->batch_ms = ms;
+  }
+  return IPUMA_OK;
+}
+
+int ipuma_synchronize(ipuma_engine *e) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (e->last_ctl) {
+    int32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, &e->last_ctl->error, 4, hipMemcpyDeviceToHost));
+    if (err) return fail(IPUMA_ETOOLONG, "a sequence is longer than %d", kMaxSeqLen);
+  }
+  return IPUMA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,761 @@
+// sw_kernels.hip -- gfx950 kernels of the batched Smith-Waterman affine-gap engine.
+//
+// Semantics (the contract every kernel here keeps bit-exact) follow the
+// reference's SWAffine vertex, src/codelets/codelet_batch_affine.cpp:37-124:
+// rows i run over B, columns j over A, H = max(diag + S[a_j][b_i], E, F, 0),
+// E = max(H_left + gI + gE, E + gE), F = max(H_up + gI + gE, F + gE), and the
+// end is the first cell in B-major order (i, then j) holding the maximum.
+//
+// Kernel families
+//   k_fast<KP>     packed int16, two comparisons per lane (low / high halves),
+//                  L lanes per comparison pipelined over the B rows (L = 1 is
+//                  the inter-sequence kernel, L = 64 the intra-sequence
+//                  anti-diagonal kernel), persistent over a queue of groups
+//                  built by the device bucketing below.
+//   k_generic32    exact int32 re-statement of the vertex loop, one comparison
+//                  per lane; re-runs int16-saturated comparisons and serves
+//                  gap settings outside the packed kernel's domain.
+//   k_classify ... device-side length bucketing (counting sort by lanes-per-
+//                  pair class, super-strips and |A|) and longest-first order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "sw_device.h"
+
+namespace ipuma {
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 U2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ s16x2 S2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ uint32_t W(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t W(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// Value of lane k-1 of the same pipeline group (lanes k = lane % L); lanes
+// with k == 0 take `own`.  DPP row_shr:1 moves within rows of 16 lanes and
+// wave_shr:1 across the wave; with bound_ctrl off the first lane of the row /
+// wave keeps `own`.
+template <int L>
+__device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool first) {
+  if constexpr (L == 1) {
+    (void)v; (void)first;
+    return own;
+  } else if constexpr (L == 64) {
+    (void)first;
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x138, 0xf, 0xf, false);
+  } else {
+    uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x111, 0xf, 0xf, false);
+    if constexpr (L == 16) {
+      (void)first;
+      return r;
+    } else {
+      return first ? own : r;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Packed int16 kernel.
+//
+// A wave runs one group: 64/L pipelines of L lanes; pipeline g holds two
+// comparisons, slot g (low halves) and slot 64/L + g (high halves), of equal
+// padded shape.  A super-strip is L*kRows rows of B; lane k of a pipeline owns
+// rows [ (ss*L + k)*kRows, +kRows ) and at step t computes column j = t - k of
+// them, receiving H/F of the row above (and the column's A code) from lane k-1
+// through DPP.  Lane 0 reads the row above the super-strip from the wave's
+// scratch, lane L-1 writes the last row back in place.
+//
+// Arithmetic is the reference recurrence with E and F clamped at 0:
+// H = max(diag + s, E', F') with E' = max(E, 0), F' = max(F, 0) is unchanged
+// because H is clamped at 0, and E'/F' follow E' <- max(H - o, E' - x, 0) when
+// o = -(gI + gE) >= 0 and x = -gE >= 0 (unsigned saturating subtraction).  The
+// add of the substitution score saturates at 32767; a comparison whose
+// maximum reaches 32767 is re-run exactly in int32.
+//
+// Padding (columns past |A|, rows past |B|, pipeline fill and drain) scores
+// the pad code at -32768, so padded cells only carry gap chains of real cells
+// and stay strictly below, or tie later in B-major order than, the real cell
+// they came from: they never move the maximum or its first position.
+//
+// Position tracking: per row, key = H << 16 | (0xFFFF - j); the max key is the
+// row maximum at its first column.  Rows merge in order with a strict >, lanes
+// merge at the end by (max H, min row).
+// ---------------------------------------------------------------------------
+template <int L, int KP>
+__device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__restrict__ tab,
+                                          const GroupDesc gd, uint32_t *__restrict__ wscr,
+                                          int lane) {
+  constexpr int R = kRows;
+  constexpr int NG = 64 / L;
+  constexpr uint32_t PADC = KP - 1;
+  constexpr uint32_t COLSTRIDE = KP * 128;  // bytes of one A-code row of the LDS table
+  const int k = lane % L;
+  const int g = lane / L;
+  const bool first = (k == 0);
+  const bool last = (k == L - 1);
+  const int mpad = gd.mpad;
+  const int nss = gd.nss;
+  const uint32_t bank4 = (uint32_t)(lane & 31) * 4u;
+
+  const int p_lo = A.sorted[gd.base + g];
+  const int p_hi = A.sorted[gd.base + NG + g];
+  int m_lo = 0, n_lo = 0, m_hi = 0, n_hi = 0;
+  const uint8_t *a_lo = A.codes, *b_lo = A.codes, *a_hi = A.codes, *b_hi = A.codes;
+  if (p_lo >= 0) {
+    m_lo = A.a_len[p_lo];
+    n_lo = A.b_len[p_lo];
+    a_lo = A.codes + A.a_off[p_lo];
+    b_lo = A.codes + A.b_off[p_lo];
+  }
+  if (p_hi >= 0) {
+    m_hi = A.a_len[p_hi];
+    n_hi = A.b_len[p_hi];
+    a_hi = A.codes + A.a_off[p_hi];
+    b_hi = A.codes + A.b_off[p_hi];
+  }
+  const uint32_t *a32_lo = (const uint32_t *)a_lo;
+  const uint32_t *a32_hi = (const uint32_t *)a_hi;
+  const uint32_t gap_o = A.gap_o, gap_x = A.gap_x;
+
+  int bh_lo = 0, bi_lo = 0, bj_lo = 0, bh_hi = 0, bi_hi = 0, bj_hi = 0;
+
+  for (int ss = 0; ss < nss; ++ss) {
+    const int row0 = (ss * L + k) * R;
+    uint32_t rp[R];
+    {
+      uint32_t wl[R / 4], wh[R / 4];
+      const uint32_t *bl = (const uint32_t *)(b_lo + row0);
+      const uint32_t *bh = (const uint32_t *)(b_hi + row0);
+#pragma unroll
+      for (int q = 0; q < R / 4; ++q) {
+        wl[q] = (row0 + 4 * q < n_lo) ? bl[q] : 0u;
+        wh[q] = (row0 + 4 * q < n_hi) ? bh[q] : 0u;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint32_t cl =
+            (row0 + r < n_lo) ? min((wl[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
+        const uint32_t ch =
+            (row0 + r < n_hi) ? min((wh[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
+        rp[r] = (cl * 128u) | ((ch * 128u) << 16);
+      }
+    }
+    uint32_t H[R], E[R], KL[R], KH[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      H[r] = 0u;
+      E[r] = 0u;
+      KL[r] = 0u;
+      KH[r] = 0u;
+    }
+    uint32_t out_h = 0u, out_f = 0u, hdiag = 0u;
+    uint32_t col_lo = PADC * COLSTRIDE, col_hi = PADC * COLSTRIDE;
+    const bool load_bnd = ss > 0;
+    const bool store_bnd = ss + 1 < nss;
+
+    uint2 nb = make_uint2(0u, 0u);
+    if (first && load_bnd && mpad > 0) nb = *(const uint2 *)(wscr + (size_t)g * 2);
+    uint32_t na_lo = 0u, na_hi = 0u;
+    if (first) {
+      na_lo = (0 < m_lo) ? a32_lo[0] : 0u;
+      na_hi = (0 < m_hi) ? a32_hi[0] : 0u;
+    }
+    const int T = mpad + L - 1;
+    for (int t0 = 0; t0 < T; t0 += 4) {
+      const uint32_t wa_lo = na_lo, wa_hi = na_hi;
+      if (first) {
+        const int tn = t0 + 4;
+        na_lo = (tn < m_lo) ? a32_lo[tn >> 2] : 0u;
+        na_hi = (tn < m_hi) ? a32_hi[tn >> 2] : 0u;
+      }
+#pragma unroll 1
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u;
+        const uint2 cb = nb;
+        if (first && load_bnd && t + 1 < mpad)
+          nb = *(const uint2 *)(wscr + ((size_t)(t + 1) * NG + g) * 2);
+        else
+          nb = make_uint2(0u, 0u);
+        const uint32_t ac_lo = (t < m_lo) ? min((wa_lo >> (8 * u)) & 0xffu, PADC) : PADC;
+        const uint32_t ac_hi = (t < m_hi) ? min((wa_hi >> (8 * u)) & 0xffu, PADC) : PADC;
+        const uint32_t rh = from_prev<L>(cb.x, out_h, first);
+        const uint32_t rf = from_prev<L>(cb.y, out_f, first);
+        col_lo = from_prev<L>(ac_lo * COLSTRIDE, col_lo, first);
+        col_hi = from_prev<L>(ac_hi * COLSTRIDE, col_hi, first);
+        const int j = t - k;
+        const uint32_t jinv = 0xFFFFu - (uint32_t)j;
+        const char *tl = (const char *)tab + (col_lo + bank4);
+        const char *th = (const char *)tab + (col_hi + bank4);
+        uint32_t sv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          // keep the row parts packed two per register across the loop
+          asm volatile("" : "+v"(rp[r]));
+          const uint32_t slo = *(const uint16_t *)(tl + (rp[r] & 0xFFFFu));
+          const uint32_t shi = *(const uint16_t *)(th + (rp[r] >> 16));
+          sv[r] = __builtin_amdgcn_perm(shi, slo, 0x05040100u);
+        }
+        // t2 of row r is diag + s; the old H of row r is consumed (as the
+        // diagonal of row r + 1) before the new H of row r is written.
+        s16x2 t2 = __builtin_elementwise_add_sat(S2(hdiag), S2(sv[0]));
+        hdiag = rh;
+        uint32_t f = rf;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          s16x2 t2n = t2;
+          if (r + 1 < R) t2n = __builtin_elementwise_add_sat(S2(H[r]), S2(sv[r + 1]));
+          const uint32_t hn =
+              W(__builtin_elementwise_max(__builtin_elementwise_max(t2, S2(E[r])), S2(f)));
+          H[r] = hn;
+          const u16x2 ho = __builtin_elementwise_sub_sat(U2(hn), U2(gap_o));
+          E[r] = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))));
+          f = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(f), U2(gap_x))));
+          KL[r] = max(KL[r], __builtin_amdgcn_perm(hn, jinv, 0x05040100u));
+          KH[r] = max(KH[r], __builtin_amdgcn_perm(hn, jinv, 0x07060100u));
+          t2 = t2n;
+        }
+        out_h = H[R - 1];
+        out_f = f;
+        if (last && store_bnd && j >= 0 && j < mpad)
+          *(uint2 *)(wscr + ((size_t)j * NG + g) * 2) = make_uint2(out_h, out_f);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int hl = (int)(KL[r] >> 16);
+      if (hl > bh_lo) {
+        bh_lo = hl;
+        bi_lo = row0 + r;
+        bj_lo = 0xFFFF - (int)(KL[r] & 0xFFFFu);
+      }
+      const int hh = (int)(KH[r] >> 16);
+      if (hh > bh_hi) {
+        bh_hi = hh;
+        bi_hi = row0 + r;
+        bj_hi = 0xFFFF - (int)(KH[r] & 0xFFFFu);
+      }
+    }
+  }
+
+  if constexpr (L > 1) {
+#pragma unroll
+    for (int off = 1; off < L; off <<= 1) {
+      const int oh = __shfl_xor(bh_lo, off), oi = __shfl_xor(bi_lo, off), oj = __shfl_xor(bj_lo, off);
+      if (oh > bh_lo || (oh == bh_lo && oi < bi_lo)) {
+        bh_lo = oh;
+        bi_lo = oi;
+        bj_lo = oj;
+      }
+      const int qh = __shfl_xor(bh_hi, off), qi = __shfl_xor(bi_hi, off), qj = __shfl_xor(bj_hi, off);
+      if (qh > bh_hi || (qh == bh_hi && qi < bi_hi)) {
+        bh_hi = qh;
+        bi_hi = qi;
+        bj_hi = qj;
+      }
+    }
+  }
+  if (first) {
+    if (p_lo >= 0) {
+      if (bh_lo >= kSat16) {
+        A.ovf_list[atomicAdd(A.ovf_count, 1)] = p_lo;
+      } else {
+        A.scores[p_lo] = bh_lo;
+        A.a_range[p_lo] = (int32_t)((uint32_t)(bj_lo & 0xFFFF) << 16);
+        A.b_range[p_lo] = (int32_t)((uint32_t)(bi_lo & 0xFFFF) << 16);
+      }
+    }
+    if (p_hi >= 0) {
+      if (bh_hi >= kSat16) {
+        A.ovf_list[atomicAdd(A.ovf_count, 1)] = p_hi;
+      } else {
+        A.scores[p_hi] = bh_hi;
+        A.a_range[p_hi] = (int32_t)((uint32_t)(bj_hi & 0xFFFF) << 16);
+        A.b_range[p_hi] = (int32_t)((uint32_t)(bi_hi & 0xFFFF) << 16);
+      }
+    }
+  }
+}
+
+template <int KP>
+__global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
+  __shared__ uint32_t tab[KP * KP * 32];
+  for (int i = threadIdx.x; i < KP * KP * 32; i += blockDim.x)
+    tab[i] = (uint32_t)(uint16_t)A.table[i >> 5];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
+  const int ng = *A.ngroups;
+  for (;;) {
+    int gi = 0;
+    if (lane == 0) gi = atomicAdd(A.queue, 1);
+    gi = __builtin_amdgcn_readfirstlane(gi);
+    if (gi >= ng) break;
+    const GroupDesc gd = A.groups[gi];
+    switch (gd.cls) {
+      case 0: run_group<1, KP>(A, tab, gd, wscr, lane); break;
+      case 1: run_group<2, KP>(A, tab, gd, wscr, lane); break;
+      case 2: run_group<4, KP>(A, tab, gd, wscr, lane); break;
+      case 3: run_group<8, KP>(A, tab, gd, wscr, lane); break;
+      case 4: run_group<16, KP>(A, tab, gd, wscr, lane); break;
+      default: run_group<64, KP>(A, tab, gd, wscr, lane); break;
+    }
+  }
+}
+
+template __global__ void k_fast<18>(FastArgs);
+template __global__ void k_fast<26>(FastArgs);
+
+// ---------------------------------------------------------------------------
+// Exact int32 kernel: the vertex loop of codelet_batch_affine.cpp:62-87 with
+// the reference's own (unclamped) recurrence and bounds, one comparison per
+// lane.  C / bG live in the wave's work area: 2048 ints per lane for |A| <=
+// 1024, the whole 2 x 65536 area for one longer comparison at a time.
+// ---------------------------------------------------------------------------
+__device__ void exact_pair(const GenericArgs &G, int p, int32_t *C, int32_t *bG) {
+  const int m = G.a_len[p], n = G.b_len[p];
+  if (m > kMaxSeqLen || n > kMaxSeqLen) {
+    atomicOr(G.error, 1);
+    G.scores[p] = -1;
+    G.a_range[p] = 0;
+    G.b_range[p] = 0;
+    return;
+  }
+  if (m <= 0 || n <= 0) {
+    G.scores[p] = 0;
+    G.a_range[p] = 0;
+    G.b_range[p] = 0;
+    return;
+  }
+  const uint8_t *a = G.codes + G.a_off[p];
+  const uint8_t *b = G.codes + G.b_off[p];
+  const int gI = G.gap_init, gE = G.gap_extend, kp = G.kp;
+  for (int j = 0; j < m; ++j) {
+    C[j] = 0;
+    bG[j] = 0;
+  }
+  int s = 0, aend = 0, bend = 0;
+  for (int i = 0; i < n; ++i) {
+    int lastNoGap = 0, prevNoGap = 0, aGap = gI;
+    const int bc = min((int)b[i], kp - 1);
+    for (int j = 0; j < m; ++j) {
+      aGap = max(lastNoGap + gI + gE, aGap + gE);
+      const int bgj = max(C[j] + gI + gE, bG[j] + gE);
+      bG[j] = bgj;
+      lastNoGap = max(prevNoGap + (int)G.table[min((int)a[j], kp - 1) * kp + bc], aGap);
+      lastNoGap = max(lastNoGap, bgj);
+      lastNoGap = max(lastNoGap, 0);
+      prevNoGap = C[j];
+      C[j] = lastNoGap;
+      if (lastNoGap > s) {
+        aend = j;
+        bend = i;
+        s = lastNoGap;
+      }
+    }
+  }
+  G.scores[p] = s;
+  G.a_range[p] = (int32_t)((uint32_t)(aend & 0xFFFF) << 16);
+  G.b_range[p] = (int32_t)((uint32_t)(bend & 0xFFFF) << 16);
+}
+
+__global__ __launch_bounds__(64) void k_generic32(GenericArgs G) {
+  const int lane = threadIdx.x;
+  int32_t *work = G.work + (int64_t)blockIdx.x * (2 * 65536);
+  const int total = G.list ? *G.count : G.n;
+  for (int base = blockIdx.x * 64; base < total; base += gridDim.x * 64) {
+    const int idx = base + lane;
+    const int p = idx < total ? (G.list ? G.list[idx] : idx) : -1;
+    const int m = p >= 0 ? G.a_len[p] : 0;
+    if (p >= 0 && (m <= 1024 || m > kMaxSeqLen)) exact_pair(G, p, work + lane * 2048, work + lane * 2048 + 1024);
+    unsigned long long longs = __ballot(p >= 0 && m > 1024 && m <= kMaxSeqLen);
+    while (longs) {
+      const int l = __ffsll((long long)longs) - 1;
+      longs &= longs - 1;
+      const int pl = __shfl(p, l);
+      if (lane == 0) exact_pair(G, pl, work, work + 65536);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small utility kernels.
+// ---------------------------------------------------------------------------
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_encode(const uint8_t *chars, int64_t len, uint8_t *codes, const uint8_t *table) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * blockDim.x)
+    codes[i] = table[chars[i]];
+}
+
+// ---------------------------------------------------------------------------
+// Device-side length bucketing (replaces partition::fillFirst / roundRobin /
+// greedy, src/partition.cpp:197-462).
+//
+// Class: lanes per comparison L = max(L_scratch, min(L_share, L_rows)) with
+//   L_scratch = ceil(|A| / 1024)   (boundary of one wave <= 512 KiB),
+//   L_share   = ceil(|A||B| / share) (cells per pipeline near the fair share
+//                                    of the launch, so big pairs get more lanes),
+//   L_rows    = ceil(|B| / kRows)  (no more lanes than row strips),
+// each rounded up to a power of two in {1, 2, 4, 8, 16, 64}.
+// Bin: (class, super-strips, |A| bucket); pairs scatter into a class-aligned
+// sorted array; groups of 128/L consecutive slots share one padded shape.
+// ---------------------------------------------------------------------------
+
+
+__device__ __forceinline__ int pow2_class(int64_t x) {
+  // smallest class whose L >= x
+  if (x <= 1) return 0;
+  if (x <= 2) return 1;
+  if (x <= 4) return 2;
+  if (x <= 8) return 3;
+  if (x <= 16) return 4;
+  return 5;
+}
+
+__device__ __forceinline__ int mbin_of(int m) {
+  return m <= 2048 ? (m + 15) / 16 : min(kMBins - 1, 128 + (m - 2048 + 511) / 512);
+}
+
+__global__ void k_sum_cells(int32_t n, const int32_t *a_len, const int32_t *b_len, BucketCtl *ctl) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long c = 0;
+  int m = 0, nb = 0;
+  if (p < n) {
+    m = a_len[p];
+    nb = b_len[p];
+    if (m > 0 && nb > 0 && m <= kMaxSeqLen && nb <= kMaxSeqLen) c = (unsigned long long)m * nb;
+    else m = nb = 0;
+  }
+  // wave reduction, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    c += __shfl_xor(c, off);
+    m = max(m, __shfl_xor(m, off));
+    nb = max(nb, __shfl_xor(nb, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (c) atomicAdd(&ctl->total_cells, c);
+    atomicMax(&ctl->maxm, m);
+    atomicMax(&ctl->maxn, nb);
+  }
+}
+
+__global__ void k_classify(int32_t n, const int32_t *a_len, const int32_t *b_len,
+                           int64_t target_slots, int32_t force_class, int32_t *bin_of,
+                           int32_t *hist, BucketCtl *ctl, int32_t *scores, int32_t *a_range,
+                           int32_t *b_range) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int m = a_len[p], nb = b_len[p];
+  if (m < 0 || nb < 0 || m > kMaxSeqLen || nb > kMaxSeqLen) {
+    atomicOr(&ctl->error, 1);
+    bin_of[p] = -1;
+    scores[p] = -1;
+    a_range[p] = 0;
+    b_range[p] = 0;
+    return;
+  }
+  if (m == 0 || nb == 0) {
+    bin_of[p] = -1;
+    scores[p] = 0;
+    a_range[p] = 0;
+    b_range[p] = 0;
+    return;
+  }
+  int cls;
+  if (force_class >= 0) {
+    // never below the class the wave scratch needs (mpad * 64 / L <= 65536)
+    cls = max(force_class, pow2_class((m + 1023) / 1024));
+  } else {
+    const int64_t share = max((int64_t)1, (int64_t)(ctl->total_cells / (unsigned long long)target_slots));
+    const int c_scr = pow2_class((m + 1023) / 1024);
+    const int c_share = pow2_class(((int64_t)m * nb + share - 1) / share);
+    const int c_rows = pow2_class((nb + kRows - 1) / kRows);
+    cls = max(c_scr, min(c_share, c_rows));
+  }
+  const int L = kClassL[cls];
+  const int nss = (nb + L * kRows - 1) / (L * kRows);
+  const int bin = (cls * kNssBins + min(nss, kNssBins - 1)) * kMBins + mbin_of(m);
+  bin_of[p] = bin;
+  atomicAdd(&hist[bin], 1);
+  atomicAdd(&ctl->class_count[cls], 1);
+}
+
+// Single block: exclusive scan of the bins, class-aligned starts, group counts.
+__global__ __launch_bounds__(1024) void k_scan_bins(int32_t *hist, BucketCtl *ctl) {
+  __shared__ int32_t part[1024];
+  __shared__ int32_t cshift[kNumClasses];
+  constexpr int PER = kNumBins / 1024;
+  static_assert(kNumBins % 1024 == 0, "bins");
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    int unaligned = 0, aligned = 0, gb = 0;
+    for (int c = 0; c < kNumClasses; ++c) {
+      const int cnt = ctl->class_count[c];
+      const int gs = 128 / kClassL[c];
+      ctl->class_start[c] = aligned;
+      ctl->class_shift[c] = aligned - unaligned;
+      cshift[c] = aligned - unaligned;
+      ctl->group_base[c] = gb;
+      gb += (cnt + gs - 1) / gs;
+      unaligned += cnt;
+      aligned += (cnt + 127) / 128 * 128;
+    }
+    ctl->group_base[kNumClasses] = gb;
+    ctl->ngroups = gb;
+  }
+  int sum = 0;
+  for (int i = 0; i < PER; ++i) sum += hist[tid * PER + i];
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;
+  for (int i = 0; i < PER; ++i) {
+    const int b = tid * PER + i;
+    const int c = b / (kNssBins * kMBins);
+    const int h = hist[b];
+    hist[b] = run + cshift[c];
+    run += h;
+  }
+}
+
+__global__ void k_scatter(int32_t n, const int32_t *bin_of, int32_t *offsets, int32_t *sorted) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int b = bin_of[p];
+  if (b < 0) return;
+  sorted[atomicAdd(&offsets[b], 1)] = p;
+}
+
+__device__ __forceinline__ int cost_bin(int64_t cost) {
+  // 16 bins per octave, largest first after the descending scan
+  if (cost <= 1) return 0;
+  const float l2 = __log2f((float)cost);
+  return min(kCostBins - 1, (int)(l2 * 16.0f));
+}
+
+__global__ void k_groups(int32_t max_groups, const int32_t *sorted, const int32_t *a_len,
+                         const int32_t *b_len, const BucketCtl *ctl, GroupDesc *gtmp,
+                         int32_t *gbin, int32_t *costhist) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= max_groups || gi >= ctl->ngroups) return;
+  int c = 0;
+  while (c + 1 < kNumClasses && gi >= ctl->group_base[c + 1]) ++c;
+  const int L = kClassL[c];
+  const int gs = 128 / L;
+  const int base = ctl->class_start[c] + (gi - ctl->group_base[c]) * gs;
+  int mpad = 0, nbmax = 0;
+  for (int s = 0; s < gs; ++s) {
+    const int p = sorted[base + s];
+    if (p >= 0) {
+      mpad = max(mpad, a_len[p]);
+      nbmax = max(nbmax, b_len[p]);
+    }
+  }
+  GroupDesc d;
+  d.cls = c;
+  d.mpad = mpad;
+  d.nss = (nbmax + L * kRows - 1) / (L * kRows);
+  d.base = base;
+  gtmp[gi] = d;
+  atomicAdd(&((BucketCtl *)ctl)->padded_cells,
+            (unsigned long long)128 * d.nss * kRows * (unsigned long long)mpad);
+  const int cb = cost_bin((int64_t)d.nss * (mpad + L - 1) * kRows);
+  gbin[gi] = cb;
+  atomicAdd(&costhist[cb], 1);
+}
+
+// Single block: descending exclusive scan of the cost bins.
+__global__ __launch_bounds__(kCostBins) void k_scan_cost(int32_t *costhist) {
+  __shared__ int32_t v[kCostBins];
+  const int tid = threadIdx.x;
+  // position tid holds bin (kCostBins - 1 - tid): largest cost first
+  const int h = costhist[kCostBins - 1 - tid];
+  v[tid] = h;
+  __syncthreads();
+  for (int off = 1; off < kCostBins; off <<= 1) {
+    const int x = tid >= off ? v[tid - off] : 0;
+    __syncthreads();
+    v[tid] += x;
+    __syncthreads();
+  }
+  costhist[kCostBins - 1 - tid] = v[tid] - h;
+}
+
+__global__ void k_order_groups(int32_t max_groups, const BucketCtl *ctl, const GroupDesc *gtmp,
+                               const int32_t *gbin, int32_t *coff, GroupDesc *groups) {
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= max_groups || gi >= ctl->ngroups) return;
+  groups[atomicAdd(&coff[gbin[gi]], 1)] = gtmp[gi];
+}
+
+// ---------------------------------------------------------------------------
+// Start positions: the reference's reverse pass (codelet_batch_affine.cpp:89-115)
+// walks i = Bend..0, j = Aend..0 with the forward recurrence; that is the
+// forward pass on the reversed prefixes A[0..Aend], B[0..Bend], whose first
+// maximum (i', j') maps back to Bstart = Bend - i', Astart = Aend - j'.
+// ---------------------------------------------------------------------------
+__global__ void k_reverse_prefix(int32_t n, const uint8_t *codes, const int64_t *a_off,
+                                 const int64_t *b_off, const int32_t *scores,
+                                 const int32_t *a_range, const int32_t *b_range,
+                                 uint8_t *rcodes, int32_t *ra_len, int32_t *rb_len) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  if (scores[p] <= 0) {
+    ra_len[p] = 0;
+    rb_len[p] = 0;
+    return;
+  }
+  const int la = (int)(((uint32_t)a_range[p] >> 16) & 0xFFFF) + 1;
+  const int lb = (int)(((uint32_t)b_range[p] >> 16) & 0xFFFF) + 1;
+  const uint8_t *a = codes + a_off[p];
+  const uint8_t *b = codes + b_off[p];
+  uint8_t *ra = rcodes + a_off[p];
+  uint8_t *rb = rcodes + b_off[p];
+  for (int t = 0; t < la; ++t) ra[t] = a[la - 1 - t];
+  for (int t = 0; t < lb; ++t) rb[t] = b[lb - 1 - t];
+  ra_len[p] = la;
+  rb_len[p] = lb;
+}
+
+__global__ void k_finish_starts(int32_t n, const int32_t *scores, const int32_t *ra_range,
+                                const int32_t *rb_range, int32_t *a_range, int32_t *b_range) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  if (scores[p] <= 0) return;
+  const uint32_t aend = ((uint32_t)a_range[p] >> 16) & 0xFFFF;
+  const uint32_t bend = ((uint32_t)b_range[p] >> 16) & 0xFFFF;
+  const uint32_t ra = ((uint32_t)ra_range[p] >> 16) & 0xFFFF;
+  const uint32_t rb = ((uint32_t)rb_range[p] >> 16) & 0xFFFF;
+  a_range[p] = (int32_t)((aend << 16) | ((aend - ra) & 0xFFFF));
+  b_range[p] = (int32_t)((bend << 16) | ((bend - rb) & 0xFFFF));
+}
+
+// ---------------------------------------------------------------------------
+// Plain padded batch: pairs in input order, one pair per lane (L = 1), every
+// group padded to the longest A and B of the whole batch.
+// ---------------------------------------------------------------------------
+__global__ void k_plain_setup(int32_t n, const int32_t *bin_of, int32_t *sorted) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sorted[i] = bin_of[i] >= 0 ? i : -1;
+}
+
+// One pair per lane unless the longest A needs a deeper pipeline for the
+// wave scratch (|A| > 1024); group g holds pairs [g*gs, (g+1)*gs) in order.
+__global__ void k_plain_groups(int32_t n, BucketCtl *ctl, GroupDesc *groups) {
+  const int cls = pow2_class((ctl->maxm + 1023) / 1024);
+  const int L = kClassL[cls];
+  const int gs = 128 / L;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ng = (n + gs - 1) / gs;
+  if (g == 0) ctl->ngroups = ng;
+  if (g >= ng) return;
+  GroupDesc d;
+  d.cls = cls;
+  d.mpad = ctl->maxm;
+  d.nss = (ctl->maxn + L * kRows - 1) / (L * kRows);
+  d.base = g * gs;
+  groups[g] = d;
+  atomicAdd(&ctl->padded_cells,
+            (unsigned long long)128 * d.nss * kRows * (unsigned long long)d.mpad);
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side).
+// ---------------------------------------------------------------------------
+static inline int nblocks(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int blocks = (int)std::min<int64_t>(nblocks(n, 256), 4096);
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_encode(const uint8_t *chars, int64_t len, uint8_t *codes, const uint8_t *table,
+                         hipStream_t st) {
+  if (len <= 0) return hipSuccess;
+  const int blocks = (int)std::min<int64_t>(nblocks(len, 256), 8192);
+  hipLaunchKernelGGL(k_encode, dim3(blocks), dim3(256), 0, st, chars, len, codes, table);
+  return hipGetLastError();
+}
+
+hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
+  const int n = B.n;
+  const int nb = nblocks(std::max(n, 1), 256);
+  hipLaunchKernelGGL(k_sum_cells, dim3(nb), dim3(256), 0, st, n, B.a_len, B.b_len, B.ctl);
+  hipLaunchKernelGGL(k_classify, dim3(nb), dim3(256), 0, st, n, B.a_len, B.b_len,
+                     B.target_slots, B.plain ? 0 : B.force_class, B.bin_of, B.hist, B.ctl,
+                     B.scores, B.a_range, B.b_range);
+  if (B.plain) {
+    hipLaunchKernelGGL(k_plain_setup, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.sorted);
+    hipLaunchKernelGGL(k_plain_groups, dim3(nblocks(n / 2 + 2, 256)), dim3(256), 0, st, n, B.ctl,
+                       B.groups);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, B.hist, B.ctl);
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.hist, B.sorted);
+  const int gb = nblocks(B.max_groups, 256);
+  hipLaunchKernelGGL(k_groups, dim3(gb), dim3(256), 0, st, B.max_groups, B.sorted, B.a_len,
+                     B.b_len, B.ctl, B.gtmp, B.gbin, B.costhist);
+  hipLaunchKernelGGL(k_scan_cost, dim3(1), dim3(kCostBins), 0, st, B.costhist);
+  hipLaunchKernelGGL(k_order_groups, dim3(gb), dim3(256), 0, st, B.max_groups, B.ctl, B.gtmp,
+                     B.gbin, B.costhist, B.groups);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast(int kp, const FastArgs &A, int blocks, hipStream_t st) {
+  if (kp == 18)
+    hipLaunchKernelGGL(k_fast<18>, dim3(blocks), dim3(kFastThreads), 0, st, A);
+  else
+    hipLaunchKernelGGL(k_fast<26>, dim3(blocks), dim3(kFastThreads), 0, st, A);
+  return hipGetLastError();
+}
+
+int fast_blocks_per_cu(int kp) {
+  int nb = 0;
+  hipError_t e = kp == 18
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<18>, kFastThreads, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<26>, kFastThreads, 0);
+  return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+hipError_t launch_generic(const GenericArgs &G, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_generic32, dim3(blocks), dim3(64), 0, st, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_reverse_prefix(int32_t n, const uint8_t *codes, const int64_t *a_off,
+                                 const int64_t *b_off, const int32_t *scores,
+                                 const int32_t *a_range, const int32_t *b_range, uint8_t *rcodes,
+                                 int32_t *ra_len, int32_t *rb_len, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_reverse_prefix, dim3(nblocks(n, 128)), dim3(128), 0, st, n, codes, a_off,
+                     b_off, scores, a_range, b_range, rcodes, ra_len, rb_len);
+  return hipGetLastError();
+}
+
+hipError_t launch_finish_starts(int32_t n, const int32_t *scores, const int32_t *ra_range,
+                                const int32_t *rb_range, int32_t *a_range, int32_t *b_range,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finish_starts, dim3(nblocks(n, 256)), dim3(256), 0, st, n, scores,
+                     ra_range, rb_range, a_range, b_range);
+  return hipGetLastError();
+}
+
+}  // namespace ipuma

@@ -1,0 +1,299 @@
+"""Python binding of the MI355X Smith-Waterman engine (ctypes over the C ABI).
+
+The C ABI (include/ipuma_hip.h) is the product boundary; this module only
+marshals arguments.  It mirrors the reference's ``ipu::batchaffine::SWAlgorithm``
+surface (src/ipu_batch_affine.h:52-118): ``SWAlgorithm(sw_config, algo_config)``,
+``compare_local(A, B)``, ``compare_mn_local(seqs, cmps)``, ``get_result()``.
+Loading fails loudly when the HIP library has not been built; there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+from functools import lru_cache
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libipuma_amd.so")
+HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "ipuma_hip.h")
+
+IPUMA_OK = 0
+ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ECONFIG", -5: "ETOOLONG",
+          -6: "EBUSY", -7: "ECLOSED"}
+
+SIMILARITY = {"na": 0, "blosum50": 1, "blosum62": 2, "simple": 3}
+DATATYPE = {"na": 0, "aa": 1, "str": 2}
+VERTEX_TYPE = {"cpp": 0, "assembly": 1, "multi": 2, "multiasm": 3}
+FILL_ALGO = {"fillfirst": 0, "roundrobin": 1, "greedy": 2}
+BATCH_BUCKETED, BATCH_PLAIN = 0, 1
+PREC_AUTO, PREC_INT32 = 0, 1
+
+
+class IpumaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _SWConfig(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "gap_init", "gap_extend", "match_value", "mismatch_value", "ambiguity_value",
+        "similarity", "datatype")]
+
+
+class _AlgoConfig(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "tiles_used", "max_ab", "max_batches", "bufsize", "vtype", "fill_algo", "forward_only",
+        "use_remote_buffer", "transmission_programs", "io_tiles")]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("pairs", ctypes.c_int64), ("cells", ctypes.c_int64),
+                ("padded_cells", ctypes.c_int64), ("groups", ctypes.c_int64),
+                ("rerun_pairs", ctypes.c_int64), ("lanes_per_pair_max", ctypes.c_int32),
+                ("rows_per_lane", ctypes.c_int32), ("fast_kernel_ms", ctypes.c_double),
+                ("batch_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+@dataclass
+class SWConfig:
+    """ipu::SWConfig (src/ipu_config.h:11-19) with the reference defaults."""
+    gapInit: int = 0
+    gapExtend: int = -1
+    matchValue: int = 1
+    mismatchValue: int = -1
+    ambiguityValue: int = -1
+    similarity: str = "na"
+    datatype: str = "na"
+
+    def _c(self):
+        return _SWConfig(self.gapInit, self.gapExtend, self.matchValue, self.mismatchValue,
+                         self.ambiguityValue, SIMILARITY[self.similarity],
+                         DATATYPE[self.datatype])
+
+
+@dataclass
+class IPUAlgoConfig:
+    """ipu::IPUAlgoConfig (src/ipu_config.h:21-31) with the reference defaults."""
+    tilesUsed: int = 1
+    maxAB: int = 300
+    maxBatches: int = 20
+    bufsize: int = 3000
+    vtype: str = "cpp"
+    fillAlgo: str = "fillfirst"
+    forwardOnly: bool = False
+    useRemoteBuffer: bool = False
+    transmissionPrograms: int = 1
+    ioTiles: int = 0
+
+    def _c(self):
+        return _AlgoConfig(self.tilesUsed, self.maxAB, self.maxBatches, self.bufsize,
+                           VERTEX_TYPE[self.vtype], FILL_ALGO[self.fillAlgo],
+                           int(self.forwardOnly), int(self.useRemoteBuffer),
+                           self.transmissionPrograms, self.ioTiles)
+
+
+def build() -> None:
+    """Compile the HIP library for gfx950 (in-tree)."""
+    subprocess.run(["make", "-s", "-j8", "-C", PKG_ROOT], check=True)
+
+
+_P = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+
+
+@lru_cache(maxsize=1)
+def lib():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {PKG_ROOT}` "
+                          "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.ipuma_last_error.restype = ctypes.c_char_p
+    L.ipuma_engine_create.argtypes = [_P, _P, ctypes.POINTER(_P)]
+    L.ipuma_engine_destroy.argtypes = [_P]
+    L.ipuma_engine_destroy.restype = None
+    L.ipuma_compare_local.argtypes = [_P, _i32, _P, _P, _P, _P, _P, _P, _P]
+    L.ipuma_compare_mn_local.argtypes = [_P, _i32, _P, _P, _i32, _P, _P, _P, _P]
+    L.ipuma_compare_device.argtypes = [_P, _i32, _P, _i64, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.ipuma_set_batching.argtypes = [_P, _i32]
+    L.ipuma_set_precision.argtypes = [_P, _i32]
+    L.ipuma_set_forward_only.argtypes = [_P, _i32]
+    L.ipuma_set_lanes_per_pair.argtypes = [_P, _i32]
+    L.ipuma_set_timing.argtypes = [_P, _i32]
+    L.ipuma_encode_device.argtypes = [_P, _P, _i64, _P, _P]
+    L.ipuma_last_batch_stats.argtypes = [_P, ctypes.POINTER(BatchStats)]
+    L.ipuma_synchronize.argtypes = [_P]
+    return L
+
+
+def _check(rc: int):
+    if rc != IPUMA_OK:
+        raise IpumaError(rc, lib().ipuma_last_error().decode(errors="replace"))
+
+
+def header_symbols(path: str = HEADER):
+    """Function names the C ABI header declares."""
+    import re
+    text = open(path).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(ipuma_\w+)\s*\(",
+                                 text, re.M)))
+
+
+def _pack(seqs):
+    enc = [s.encode("latin-1") if isinstance(s, str) else bytes(s) for s in seqs]
+    offs = np.zeros(len(enc) + 1, np.int64)
+    if enc:
+        offs[1:] = np.cumsum([len(x) for x in enc])
+    buf = b"".join(enc) + b"\0"
+    return buf, offs
+
+
+@dataclass
+class BlockAlignmentResults:
+    """ipu::batchaffine::BlockAlignmentResults (src/ipu_batch_affine.h:22-26)."""
+    scores: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    a_range_result: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    b_range_result: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+
+
+class SWAlgorithm:
+    """Engine handle with the reference's SWAlgorithm calling convention."""
+
+    def __init__(self, config: SWConfig | None = None, algoconfig: IPUAlgoConfig | None = None):
+        self.config = config or SWConfig()
+        self.algoconfig = algoconfig or IPUAlgoConfig()
+        h = _P()
+        sw = self.config._c()
+        al = self.algoconfig._c()
+        _check(lib().ipuma_engine_create(ctypes.byref(sw), ctypes.byref(al), ctypes.byref(h)))
+        self._h = h
+        n = self.algoconfig.tilesUsed * self.algoconfig.maxBatches
+        self._result = BlockAlignmentResults(np.zeros(n, np.int32), np.zeros(n, np.int32),
+                                             np.zeros(n, np.int32))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ipuma_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- policies -----------------------------------------------------------
+    def set_batching(self, mode: int):
+        _check(lib().ipuma_set_batching(self._h, mode))
+
+    def set_precision(self, mode: int):
+        _check(lib().ipuma_set_precision(self._h, mode))
+
+    def set_forward_only(self, flag: bool):
+        _check(lib().ipuma_set_forward_only(self._h, int(bool(flag))))
+
+    def set_timing(self, on: bool):
+        _check(lib().ipuma_set_timing(self._h, int(bool(on))))
+
+    def set_lanes_per_pair(self, lanes: int):
+        _check(lib().ipuma_set_lanes_per_pair(self._h, int(lanes)))
+
+    # --- reference surface --------------------------------------------------
+    def _store(self, s, a, b):
+        n = len(s)
+        cap = max(len(self._result.scores), n)
+        for name, v in (("scores", s), ("a_range_result", a), ("b_range_result", b)):
+            arr = getattr(self._result, name)
+            if len(arr) < cap:
+                arr = np.concatenate([arr, np.zeros(cap - len(arr), np.int32)])
+            arr[:n] = v
+            setattr(self._result, name, arr)
+
+    def compare_local(self, A, B, errcheck: bool = False):
+        """compare_local (src/ipu_batch_affine.cpp:564-663)."""
+        if len(A) != len(B):
+            raise ValueError("A and B must have the same number of sequences")
+        s, a, b = self.compare(A, B)
+        self._store(s, a, b)
+
+    def compare_mn_local(self, seqs, cmps, errcheck: bool = False):
+        """compare_mn_local (src/ipu_batch_affine.cpp:697-750)."""
+        s, a, b = self.compare_mn(seqs, cmps)
+        self._store(s, a, b)
+
+    def get_result(self) -> BlockAlignmentResults:
+        """get_result (src/ipu_batch_affine.cpp:408): copies sized tilesUsed*maxBatches."""
+        r = self._result
+        return BlockAlignmentResults(r.scores.copy(), r.a_range_result.copy(),
+                                     r.b_range_result.copy())
+
+    # --- array API ----------------------------------------------------------
+    def compare(self, A, B):
+        n = len(A)
+        abuf, aoff = _pack(A)
+        bbuf, boff = _pack(B)
+        s = np.zeros(n, np.int32)
+        ar = np.zeros(n, np.int32)
+        br = np.zeros(n, np.int32)
+        _check(lib().ipuma_compare_local(
+            self._h, n, abuf, aoff.ctypes.data, bbuf, boff.ctypes.data, s.ctypes.data,
+            ar.ctypes.data, br.ctypes.data))
+        return s, ar, br
+
+    def compare_mn(self, seqs, cmps):
+        buf, offs = _pack(seqs)
+        c = np.ascontiguousarray(np.asarray(cmps, dtype=np.int32).reshape(-1, 2))
+        n = c.shape[0]
+        s = np.zeros(n, np.int32)
+        ar = np.zeros(n, np.int32)
+        br = np.zeros(n, np.int32)
+        _check(lib().ipuma_compare_mn_local(
+            self._h, len(seqs), buf, offs.ctypes.data, n, c.ctypes.data, s.ctypes.data,
+            ar.ctypes.data, br.ctypes.data))
+        return s, ar, br
+
+    def compare_device(self, codes, codes_len, a_off, a_len, b_off, b_len, scores, a_range,
+                       b_range, stream=None):
+        """Device batch; every argument is a device pointer (int) or a tensor."""
+        def ptr(x):
+            return x if isinstance(x, int) else x.data_ptr()
+        n = int(a_len.numel()) if hasattr(a_len, "numel") else int(len(a_len))
+        _check(lib().ipuma_compare_device(
+            self._h, n, ptr(codes), int(codes_len), ptr(a_off), ptr(a_len), ptr(b_off),
+            ptr(b_len), ptr(scores), ptr(a_range), ptr(b_range), stream))
+
+    def encode_device(self, chars, length, codes, stream=None):
+        def ptr(x):
+            return x if isinstance(x, int) else x.data_ptr()
+        _check(lib().ipuma_encode_device(self._h, ptr(chars), int(length), ptr(codes), stream))
+
+    def stats(self) -> dict:
+        st = BatchStats()
+        _check(lib().ipuma_last_batch_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def synchronize(self):
+        _check(lib().ipuma_synchronize(self._h))
+
+
+def unpack_range(x):
+    """(begin, end) the way the reference tests unpack (test/simple-test.cpp:64-70)."""
+    x = np.asarray(x).astype(np.int64) & 0xFFFFFFFF
+    begin = (x & 0xFFFF).astype(np.uint16).astype(np.int16)
+    end = (x >> 16).astype(np.uint16).astype(np.int16)
+    return begin, end

@@ -1,0 +1,235 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle: bit-exact
+scores and packed A/B ranges on the reference's fixtures and on seeded batches."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import ipuma_amd as I
+from ipuma_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+BIG = I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1)
+SIM = {"na": O.SIM_NA, "blosum50": O.SIM_BLOSUM50, "blosum62": O.SIM_BLOSUM62}
+DT = {"na": O.DT_NA, "aa": O.DT_AA}
+
+
+def note(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def oracle_batch(batch, forward_only):
+    sw = batch["sw"]
+    mat = O.select_matrix(SIM[sw["similarity"]], sw["matchValue"], sw["mismatchValue"],
+                          sw["ambiguityValue"])
+    return O.sw_batch_codes(batch["codes"], batch["a_off"], batch["a_len"], batch["b_off"],
+                            batch["b_len"], mat, sw["gapInit"], sw["gapExtend"], forward_only)
+
+
+def engine_for(sw: dict, forward_only=False):
+    return I.SWAlgorithm(I.SWConfig(**sw), I.IPUAlgoConfig(
+        tilesUsed=BIG.tilesUsed, maxAB=BIG.maxAB, maxBatches=1, forwardOnly=forward_only))
+
+
+def device_run(eng, batch):
+    import torch
+    dev = torch.device("cuda:0")
+    codes = torch.from_numpy(batch["codes"]).to(dev)
+    a_off = torch.from_numpy(batch["a_off"]).to(dev)
+    b_off = torch.from_numpy(batch["b_off"]).to(dev)
+    a_len = torch.from_numpy(batch["a_len"]).to(dev)
+    b_len = torch.from_numpy(batch["b_len"]).to(dev)
+    n = len(batch["a_len"])
+    s = torch.empty(n, dtype=torch.int32, device=dev)
+    ar = torch.empty(n, dtype=torch.int32, device=dev)
+    br = torch.empty(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    eng.compare_device(codes, codes.numel(), a_off, a_len, b_off, b_len, s, ar, br)
+    eng.synchronize()
+    return s.cpu().numpy(), ar.cpu().numpy(), br.cpu().numpy()
+
+
+def assert_same(got, exp, label):
+    names = ("scores", "a_range", "b_range")
+    for g, e, nm in zip(got, exp, names):
+        g = np.asarray(g)[: len(e)]
+        bad = np.nonzero(g != e)[0]
+        assert bad.size == 0, (f"{label}: {nm} differ at {bad.size} pairs, first {bad[:5]}: "
+                               f"got {g[bad[:5]]} want {e[bad[:5]]}")
+
+
+def test_simple_golden_rows(gpu):
+    note("simple-test golden rows via compare_local")
+    d = load("simple_test.json")
+    A = [r[1] for r in d["rows"]]
+    B = [r[0] for r in d["rows"]]
+    eng = I.SWAlgorithm(I.SWConfig(), I.IPUAlgoConfig(tilesUsed=2, maxAB=300, maxBatches=20))
+    eng.compare_local(A, B)
+    res = eng.get_result()
+    assert len(res.scores) == 40  # tilesUsed * maxBatches, like the reference
+    qb, qe = I.unpack_range(res.a_range_result)
+    rb, re_ = I.unpack_range(res.b_range_result)
+    for k, (_, _, qs, qend, rs, rend, score) in enumerate(d["rows"]):
+        assert (int(res.scores[k]), int(qb[k]), int(qe[k]), int(rb[k]), int(re_[k])) == \
+            (score, qs, qend, rs, rend), k
+
+
+def test_mn_comparisons(gpu):
+    note("mn_cmp-test expected scores via compare_mn_local")
+    d = load("mn_test.json")
+    eng = I.SWAlgorithm(I.SWConfig(), I.IPUAlgoConfig(tilesUsed=1, maxAB=20, maxBatches=30))
+    eng.compare_mn_local(d["seqs"], d["comparisons"])
+    assert list(map(int, eng.get_result().scores[:4])) == d["expected_scores"]
+
+
+def test_simple_rows_as_mn(gpu):
+    # test/simple-test.cpp:150-171: the same rows through the MN path
+    note("simple rows through compare_mn_local")
+    d = load("simple_test.json")
+    seqs, cmps = [], []
+    for i, r in enumerate(d["rows"]):
+        seqs += [r[1], r[0]]
+        cmps.append((2 * i, 2 * i + 1))
+    eng = I.SWAlgorithm(I.SWConfig(), I.IPUAlgoConfig(tilesUsed=4, maxAB=300, maxBatches=5))
+    eng.compare_mn_local(seqs, cmps)
+    res = eng.get_result()
+    for k, r in enumerate(d["rows"]):
+        assert int(res.scores[k]) == r[6]
+        assert tuple(map(int, I.unpack_range(res.a_range_result[k:k + 1]))) == (r[2], r[3])
+
+
+def test_ssw_fixtures(gpu):
+    d = load("ssw_fixtures.json")
+    names = {0: "na", 1: "blosum50", 2: "blosum62"}
+    for case in d["cases"]:
+        note(f"ssw fixture {case['name']}")
+        c = case["config"]
+        sw = dict(gapInit=c["gap_init"], gapExtend=c["gap_extend"], matchValue=c["match"],
+                  mismatchValue=c["mismatch"], ambiguityValue=c["ambiguity"],
+                  similarity=names[c["similarity"]], datatype="na" if c["datatype"] == 0 else "aa")
+        eng = engine_for(sw)
+        A = [p[0] for p in case["pairs"]]
+        B = [p[1] for p in case["pairs"]]
+        s, ar, br = eng.compare(A, B)
+        qb, qe = I.unpack_range(ar)
+        rb, re_ = I.unpack_range(br)
+        for k, (score, eqb, eqe, erb, ere) in enumerate(case["expected"]):
+            assert int(s[k]) == score, (case["name"], k)
+            if score > 0:
+                assert (int(qb[k]), int(qe[k]), int(rb[k]), int(re_[k])) == (eqb, eqe, erb, ere), \
+                    (case["name"], k)
+
+
+@pytest.mark.parametrize("protein", [False, True])
+@pytest.mark.parametrize("mode", [I.BATCH_BUCKETED, I.BATCH_PLAIN])
+@pytest.mark.parametrize("forward_only", [True, False])
+def test_random_batches(gpu, protein, mode, forward_only):
+    note(f"random batch protein={protein} mode={mode} forward_only={forward_only}")
+    batch = W.random_pairs(100 + protein * 7 + mode * 3, 700, (1, 400), (1, 400), protein=protein)
+    eng = engine_for(batch["sw"], forward_only)
+    eng.set_batching(mode)
+    got = device_run(eng, batch)
+    assert_same(got, oracle_batch(batch, forward_only), "random")
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 64])
+def test_every_pipeline_depth(gpu, lanes):
+    note(f"lanes per pair {lanes}")
+    batch = W.random_pairs(300 + lanes, 300, (1, 700), (1, 1500), protein=lanes % 2 == 0)
+    eng = engine_for(batch["sw"], False)
+    eng.set_lanes_per_pair(lanes)
+    got = device_run(eng, batch)
+    assert_same(got, oracle_batch(batch, False), f"lanes={lanes}")
+
+
+def test_config0_bucketed_and_plain(gpu):
+    note("dna plain config (1024 pairs) bucketed vs plain vs oracle")
+    batch = W.config0()
+    exp = oracle_batch(batch, False)
+    for mode in (I.BATCH_BUCKETED, I.BATCH_PLAIN):
+        eng = engine_for(batch["sw"], False)
+        eng.set_batching(mode)
+        assert_same(device_run(eng, batch), exp, f"config0 mode={mode}")
+
+
+def test_protein_slice_forward(gpu):
+    note("protein config slice (4000 pairs) forward only")
+    batch = W.config1(n=4000)
+    eng = engine_for(batch["sw"], True)
+    assert_same(device_run(eng, batch), oracle_batch(batch, True), "protein slice")
+
+
+def test_long_pairs(gpu):
+    note("long DNA pairs (2k-6k) with starts")
+    rng = np.random.default_rng(9)
+    pairs = []
+    for L in (2000, 3500, 6000):
+        a = W._dna_draw(rng, L)
+        pairs.append((a, W._mutate(rng, a, 0.13, 0.02, W._dna_draw, W._dna_subst)))
+    batch = W.make_batch(pairs, "long", W.DNA_SW)
+    eng = engine_for(batch["sw"], False)
+    assert_same(device_run(eng, batch), oracle_batch(batch, False), "long")
+
+
+def test_int16_saturation_rerun(gpu):
+    # W^L vs W^L under BLOSUM62 scores 11*L: 45056 at L = 4096 overflows int16
+    note("int16 saturation -> int32 re-run (W^4096)")
+    batch = W.make_batch([(np.full(4096, 17, np.uint8), np.full(4096, 17, np.uint8)),
+                          (np.full(3000, 17, np.uint8), np.full(2990, 17, np.uint8)),
+                          (np.full(100, 17, np.uint8), np.full(100, 17, np.uint8))],
+                         "sat", W.PROTEIN_SW)
+    eng = engine_for(batch["sw"], False)
+    got = device_run(eng, batch)
+    assert int(got[0][0]) == 11 * 4096
+    assert eng.stats()["rerun_pairs"] >= 1
+    assert_same(got, oracle_batch(batch, False), "saturation")
+
+
+def test_int32_mode_matches(gpu):
+    note("int32 precision mode")
+    batch = W.random_pairs(77, 200, (1, 300), (1, 300), protein=True)
+    eng = engine_for(batch["sw"], False)
+    eng.set_precision(I.PREC_INT32)
+    assert_same(device_run(eng, batch), oracle_batch(batch, False), "int32 mode")
+
+
+def test_empty_and_tiny(gpu):
+    note("empty and tiny sequences")
+    eng = engine_for(W.DNA_SW, False)
+    A = ["", "A", "", "ACGT", "T", "G" * 5]
+    B = ["ACGT", "", "", "A", "T", "G"]
+    s, ar, br = eng.compare(A, B)
+    exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1)
+    assert_same((s, ar, br), exp, "empty")
+
+
+def test_gap_settings_outside_packed_domain(gpu):
+    # a positive gap extension is outside the packed kernel; the exact int32
+    # kernel takes over and still matches the reference recurrence
+    note("positive gap extension -> exact int32 path")
+    sw = dict(W.DNA_SW, gapInit=-3, gapExtend=1)
+    batch = W.random_pairs(55, 100, (1, 60), (1, 60))
+    batch["sw"] = sw
+    eng = engine_for(sw, False)
+    assert_same(device_run(eng, batch), oracle_batch(batch, False), "gap domain")
+
+
+def test_errors(gpu):
+    note("error paths")
+    eng = I.SWAlgorithm(I.SWConfig(), I.IPUAlgoConfig(tilesUsed=1, maxAB=10, maxBatches=2))
+    with pytest.raises(I.IpumaError):
+        eng.compare_local(["A"] * 3, ["A"] * 3)  # more than tilesUsed*maxBatches
+    with pytest.raises(I.IpumaError):
+        eng.compare_local(["A" * 11], ["A"])  # |A| > maxAB
+    with pytest.raises(I.IpumaError):
+        I.SWAlgorithm(I.SWConfig(similarity="simple"), I.IPUAlgoConfig())
