@@ -42,7 +42,8 @@ struct BucketCtl {
   int32_t class_shift[kNumClasses];  // aligned start - unaligned prefix
   int32_t group_base[kNumClasses + 1];
   int32_t ngroups;
-  int32_t error;       // bit 0: a length outside [0, kMaxSeqLen]
+  int32_t error;       // bit 0: a length outside [0, kMaxSeqLen]; bit 1: an end
+                       // position outside its sequence; bit 2: a bad group
   int32_t ovf_count;   // int16-saturated comparisons queued for int32
   int32_t queue;       // persistent-kernel group counter
   int32_t maxm;        // plain batching: longest A
@@ -70,6 +71,8 @@ struct FastArgs {
   int32_t *b_range;
   int32_t *ovf_list;           // pairs that saturated int16
   int32_t *ovf_count;
+  int32_t *error;              // bit 2: a group descriptor out of range
+  int32_t max_groups;          // bound of *ngroups
   uint32_t gap_o;              // -(gapInit + gapExtend), packed twice
   uint32_t gap_x;              // -gapExtend, packed twice
 };
@@ -85,7 +88,7 @@ struct GenericArgs {
   int32_t kp;
   const int32_t *list;   // pair ids, or nullptr for 0..n-1
   const int32_t *count;  // device count of `list`, or nullptr
-  int32_t n;             // pair count when list == nullptr
+  int32_t n;             // pair count (bound of *count when list != nullptr)
   int32_t *work;         // 2 * 65536 ints per wave
   int32_t *error;        // set to 1 when a length exceeds kMaxSeqLen
   int32_t gap_init;
@@ -130,9 +133,11 @@ hipError_t launch_fast(int kp, const FastArgs &A, int blocks, hipStream_t st);
 int fast_blocks_per_cu(int kp);
 hipError_t launch_generic(const GenericArgs &G, int blocks, hipStream_t st);
 hipError_t launch_reverse_prefix(int32_t n, const uint8_t *codes, const int64_t *a_off,
-                                 const int64_t *b_off, const int32_t *scores,
+                                 const int32_t *a_len, const int64_t *b_off,
+                                 const int32_t *b_len, const int32_t *scores,
                                  const int32_t *a_range, const int32_t *b_range, uint8_t *rcodes,
-                                 int32_t *ra_len, int32_t *rb_len, hipStream_t st);
+                                 int32_t *ra_len, int32_t *rb_len, int32_t *error,
+                                 hipStream_t st);
 hipError_t launch_finish_starts(int32_t n, const int32_t *scores, const int32_t *ra_range,
                                 const int32_t *rb_range, int32_t *a_range, int32_t *b_range,
                                 hipStream_t st);

@@ -209,6 +209,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   A.ngroups = &ctl->ngroups; A.queue = &ctl->queue; A.scratch = e->scratch.as<uint32_t>();
   A.scores = scores; A.a_range = a_range; A.b_range = b_range;
   A.ovf_list = c.ovf.as<int32_t>(); A.ovf_count = &ctl->ovf_count;
+  A.error = &ctl->error; A.max_groups = n + 64;
   const uint32_t o = (uint32_t)(-(e->sw.gap_init + e->sw.gap_extend)) & 0xFFFFu;
   const uint32_t x = (uint32_t)(-e->sw.gap_extend) & 0xFFFFu;
   A.gap_o = o | (o << 16);
@@ -220,7 +221,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   GenericArgs G{};
   G.codes = codes; G.a_off = a_off; G.a_len = a_len; G.b_off = b_off; G.b_len = b_len;
   G.table = e->d_table; G.kp = e->kp; G.list = c.ovf.as<int32_t>(); G.count = &ctl->ovf_count;
-  G.n = 0; G.work = e->work.as<int32_t>(); G.gap_init = e->sw.gap_init;
+  G.n = n; G.work = e->work.as<int32_t>(); G.gap_init = e->sw.gap_init;
   G.gap_extend = e->sw.gap_extend; G.scores = scores; G.a_range = a_range; G.b_range = b_range;
   G.error = &ctl->error;
   HIP_TRY(launch_generic(G, e->generic_blocks, st));
@@ -250,8 +251,9 @@ int compare_device_impl(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_
     return rc;
   int32_t *ra_len = e->rlen.as<int32_t>(), *rb_len = ra_len + n;
   int32_t *rs = e->rres.as<int32_t>(), *rar = rs + n, *rbr = rs + 2 * (size_t)n;
-  HIP_TRY(launch_reverse_prefix(n, codes, a_off, b_off, scores, a_range, b_range,
-                                e->rcodes.as<uint8_t>(), ra_len, rb_len, st));
+  HIP_TRY(launch_reverse_prefix(n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range,
+                                e->rcodes.as<uint8_t>(), ra_len, rb_len,
+                                &e->fwd.ctl.as<BucketCtl>()->error, st));
   rc = run_chain(e, e->rev, n, e->rcodes.as<uint8_t>(), a_off, ra_len, b_off, rb_len, rs, rar, rbr,
                  st);
   if (rc) return rc;
@@ -285,6 +287,21 @@ int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
   return IPUMA_OK;
 }
 
+// Error bits the kernels raise in the chains' control blocks.
+int check_device_error(ipuma_engine *e) {
+  int32_t err = 0;
+  for (ChainBufs *c : {&e->fwd, &e->rev}) {
+    if (!c->ctl.p) continue;
+    int32_t v = 0;
+    HIP_TRY(hipMemcpy(&v, &c->ctl.as<BucketCtl>()->error, 4, hipMemcpyDeviceToHost));
+    err |= v;
+  }
+  if (err & 1) return fail(IPUMA_ETOOLONG, "a sequence is longer than %d", kMaxSeqLen);
+  if (err & 2) return fail(IPUMA_EDEVICE, "internal: an end position outside its sequence");
+  if (err & 4) return fail(IPUMA_EDEVICE, "internal: a work-group descriptor out of range");
+  return IPUMA_OK;
+}
+
 int run_host_batch(ipuma_engine *e, int32_t n, int64_t codes_len, int32_t *scores,
                    int32_t *a_range, int32_t *b_range) {
   int rc;
@@ -312,10 +329,7 @@ int run_host_batch(ipuma_engine *e, int32_t n, int64_t codes_len, int32_t *score
     HIP_TRY(hipMemcpyAsync(b_range, d_br, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
   }
   HIP_TRY(hipStreamSynchronize(e->stream));
-  int32_t err = 0;
-  HIP_TRY(hipMemcpy(&err, &e->fwd.ctl.as<BucketCtl>()->error, 4, hipMemcpyDeviceToHost));
-  if (err) return fail(IPUMA_ETOOLONG, "a sequence is longer than %d", kMaxSeqLen);
-  return IPUMA_OK;
+  return check_device_error(e);
 }
 
 int check_sizes(ipuma_engine *e, int32_t n, const int64_t *a_lens_or_null,
@@ -574,12 +588,8 @@ int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out) {
 int ipuma_synchronize(ipuma_engine *e) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
   HIP_TRY(hipStreamSynchronize(e->stream));
-  if (e->last_ctl) {
-    int32_t err = 0;
-    HIP_TRY(hipMemcpy(&err, &e->last_ctl->error, 4, hipMemcpyDeviceToHost));
-    if (err) return fail(IPUMA_ETOOLONG, "a sequence is longer than %d", kMaxSeqLen);
-  }
-  return IPUMA_OK;
+  HIP_TRY(hipDeviceSynchronize());
+  return check_device_error(e);
 }
 
 }  // extern "C"

@@ -123,6 +123,9 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   int bh_lo = 0, bi_lo = 0, bj_lo = 0, bh_hi = 0, bi_hi = 0, bj_hi = 0;
 
   for (int ss = 0; ss < nss; ++ss) {
+    // The boundary row written by lane L-1 during the previous super-strip is
+    // read back by lane 0: complete those stores and drop stale L1 lines.
+    if (ss > 0) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
     const int row0 = (ss * L + k) * R;
     uint32_t rp[R];
     {
@@ -288,13 +291,26 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   const int lane = threadIdx.x & 63;
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
-  const int ng = *A.ngroups;
+  const int ng = min(*A.ngroups, A.max_groups);
   for (;;) {
     int gi = 0;
     if (lane == 0) gi = atomicAdd(A.queue, 1);
     gi = __builtin_amdgcn_readfirstlane(gi);
     if (gi >= ng) break;
     const GroupDesc gd = A.groups[gi];
+    // a descriptor outside what the bucketing can produce (pipeline class,
+    // scratch row mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and
+    // skipped rather than run
+    bool ok = gd.cls >= 0 && gd.cls < kNumClasses && gd.base >= 0;
+    if (ok) {
+      const int L = kClassL[gd.cls];
+      ok = gd.mpad >= 0 && gd.mpad <= kMaxSeqLen && (int64_t)gd.mpad * (64 / L) <= 65536 &&
+           gd.nss >= 0 && (int64_t)(gd.nss - 1) * L * kRows < kMaxSeqLen;
+    }
+    if (!ok) {
+      if (lane == 0) atomicOr(A.error, 4);
+      continue;
+    }
     switch (gd.cls) {
       case 0: run_group<1, KP>(A, tab, gd, wscr, lane); break;
       case 1: run_group<2, KP>(A, tab, gd, wscr, lane); break;
@@ -365,7 +381,7 @@ __device__ void exact_pair(const GenericArgs &G, int p, int32_t *C, int32_t *bG)
 __global__ __launch_bounds__(64) void k_generic32(GenericArgs G) {
   const int lane = threadIdx.x;
   int32_t *work = G.work + (int64_t)blockIdx.x * (2 * 65536);
-  const int total = G.list ? *G.count : G.n;
+  const int total = G.list ? min(*G.count, G.n) : G.n;
   for (int base = blockIdx.x * 64; base < total; base += gridDim.x * 64) {
     const int idx = base + lane;
     const int p = idx < total ? (G.list ? G.list[idx] : idx) : -1;
@@ -609,18 +625,23 @@ __global__ void k_order_groups(int32_t max_groups, const BucketCtl *ctl, const G
 // maximum (i', j') maps back to Bstart = Bend - i', Astart = Aend - j'.
 // ---------------------------------------------------------------------------
 __global__ void k_reverse_prefix(int32_t n, const uint8_t *codes, const int64_t *a_off,
-                                 const int64_t *b_off, const int32_t *scores,
+                                 const int32_t *a_len, const int64_t *b_off,
+                                 const int32_t *b_len, const int32_t *scores,
                                  const int32_t *a_range, const int32_t *b_range,
-                                 uint8_t *rcodes, int32_t *ra_len, int32_t *rb_len) {
+                                 uint8_t *rcodes, int32_t *ra_len, int32_t *rb_len,
+                                 int32_t *error) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  if (scores[p] <= 0) {
-    ra_len[p] = 0;
-    rb_len[p] = 0;
-    return;
-  }
+  ra_len[p] = 0;
+  rb_len[p] = 0;
+  if (scores[p] <= 0) return;
   const int la = (int)(((uint32_t)a_range[p] >> 16) & 0xFFFF) + 1;
   const int lb = (int)(((uint32_t)b_range[p] >> 16) & 0xFFFF) + 1;
+  // an end outside its sequence would write past the reversed copy
+  if (la > a_len[p] || lb > b_len[p]) {
+    atomicOr(error, 2);
+    return;
+  }
   const uint8_t *a = codes + a_off[p];
   const uint8_t *b = codes + b_off[p];
   uint8_t *ra = rcodes + a_off[p];
@@ -740,12 +761,14 @@ hipError_t launch_generic(const GenericArgs &G, int blocks, hipStream_t st) {
 }
 
 hipError_t launch_reverse_prefix(int32_t n, const uint8_t *codes, const int64_t *a_off,
-                                 const int64_t *b_off, const int32_t *scores,
+                                 const int32_t *a_len, const int64_t *b_off,
+                                 const int32_t *b_len, const int32_t *scores,
                                  const int32_t *a_range, const int32_t *b_range, uint8_t *rcodes,
-                                 int32_t *ra_len, int32_t *rb_len, hipStream_t st) {
+                                 int32_t *ra_len, int32_t *rb_len, int32_t *error,
+                                 hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_reverse_prefix, dim3(nblocks(n, 128)), dim3(128), 0, st, n, codes, a_off,
-                     b_off, scores, a_range, b_range, rcodes, ra_len, rb_len);
+                     a_len, b_off, b_len, scores, a_range, b_range, rcodes, ra_len, rb_len, error);
   return hipGetLastError();
 }
 

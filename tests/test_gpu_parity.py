@@ -233,3 +233,15 @@ def test_errors(gpu):
         eng.compare_local(["A" * 11], ["A"])  # |A| > maxAB
     with pytest.raises(I.IpumaError):
         I.SWAlgorithm(I.SWConfig(similarity="simple"), I.IPUAlgoConfig())
+
+
+def test_cpp_facade(gpu):
+    # ipu::batchaffine::SWAlgorithm (C++) over the C ABI: the reference's gtest
+    # cases re-hosted in tests/cpp/facade_test.cpp
+    import subprocess
+    note("C++ facade test binary")
+    exe = os.path.join(os.path.dirname(I.LIB_PATH), "facade_test")
+    assert os.path.exists(exe), "build with `make -C ipuma-lib_amd`"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]

@@ -1,0 +1,136 @@
+"""CPU model of the packed kernel's schedule, checked against the oracle.
+
+It replays, for one comparison, what k_fast does lane by lane: super-strips of
+L*R rows, lane k computing column j = t - k at step t, H/F/code handed to lane
+k+1 each step, the row above each super-strip read from / written to a
+boundary row, E and F clamped at 0 with unsigned saturating subtraction, the
+substitution add saturating at 32767, pad code scores of -32768, per-row keys
+H << 16 | (0xFFFF - j), rows merged with a strict > and lanes by (max H, min
+row).  Agreement with the oracle on random inputs -- including pipeline fill
+and drain, padded rows and columns, and the saturation flag -- checks the
+algebra of the kernel before it ever runs on a device.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+SAT = 32767
+
+
+def model(a, b, mat, gI, gE, L, R, mpad=None):
+    m, n = len(a), len(b)
+    o, x = -(gI + gE), -gE
+    dim = mat.shape[0]
+    pad = dim
+    table = np.full((dim + 1, dim + 1), -32768, np.int64)
+    table[:dim, :dim] = mat
+    mpad = m if mpad is None else mpad
+    nss = -(-n // (L * R))
+    bnd_h = np.zeros(mpad, np.int64)
+    bnd_f = np.zeros(mpad, np.int64)
+    best = [(0, 0, 0)] * L  # per lane (h, i, j)
+
+    def subsat(v, c):
+        return max(v - c, 0)
+
+    for ss in range(nss):
+        rows = [[(ss * L + k) * R + r for r in range(R)] for k in range(L)]
+        bcode = [[b[i] if i < n else pad for i in rows[k]] for k in range(L)]
+        H = np.zeros((L, R), np.int64)
+        E = np.zeros((L, R), np.int64)
+        key = np.zeros((L, R), np.int64)
+        out_h = [0] * L
+        out_f = [0] * L
+        col = [pad] * L
+        hdiag = [0] * L
+        new_bh = bnd_h.copy()
+        new_bf = bnd_f.copy()
+        T = mpad + L - 1
+        T = -(-T // 4) * 4
+        for t in range(T):
+            prev_h, prev_f, prev_c = out_h[:], out_f[:], col[:]
+            for k in range(L):
+                if k == 0:
+                    rh = bnd_h[t] if (ss > 0 and t < mpad) else 0
+                    rf = bnd_f[t] if (ss > 0 and t < mpad) else 0
+                    col[k] = a[t] if t < m else pad
+                else:
+                    rh, rf = prev_h[k - 1], prev_f[k - 1]
+                    col[k] = prev_c[k - 1]
+                j = t - k
+                jinv = (0xFFFF - j) & 0xFFFF
+                d = hdiag[k]
+                hdiag[k] = rh
+                f = rf
+                for r in range(R):
+                    s = table[col[k], bcode[k][r]]
+                    t2 = min(d + s, SAT)
+                    hn = max(t2, E[k, r], f)
+                    ho = subsat(hn, o)
+                    E[k, r] = max(ho, subsat(E[k, r], x))
+                    f = max(ho, subsat(f, x))
+                    key[k, r] = max(key[k, r], (hn << 16) | jinv)
+                    d = H[k, r]
+                    H[k, r] = hn
+                out_h[k] = int(H[k, R - 1])
+                out_f[k] = int(f)
+                if k == L - 1 and 0 <= j < mpad:
+                    new_bh[j] = out_h[k]
+                    new_bf[j] = out_f[k]
+        bnd_h, bnd_f = new_bh, new_bf
+        for k in range(L):
+            for r in range(R):
+                h = int(key[k, r] >> 16)
+                if h > best[k][0]:
+                    best[k] = (h, rows[k][r], 0xFFFF - int(key[k, r] & 0xFFFF))
+    h, i, j = 0, 0, 0
+    for bh, bi, bj in best:
+        if bh > h or (bh == h and bi < i):
+            h, i, j = bh, bi, bj
+    return h, i, j
+
+
+@pytest.mark.parametrize("L,R", [(1, 4), (2, 4), (4, 2), (3, 3), (8, 1)])
+def test_model_matches_oracle_dna(L, R):
+    rng = np.random.default_rng(L * 10 + R)
+    mat = O.select_matrix(O.SIM_NA, 1, -1, -1)
+    for trial in range(25):
+        m, n = rng.integers(1, 30, 2)
+        a = rng.choice([1, 2, 3, 4], m).astype(np.int64)
+        b = rng.choice([1, 2, 3, 4], n).astype(np.int64)
+        if trial % 2:
+            b = np.concatenate([b[: n // 2], a, b[n // 2:]])
+        gI, gE = [(0, -1), (-2, -1), (-4, -2)][trial % 3]
+        mpad = m + int(rng.integers(0, 9))  # group padding
+        h, i, j = model(a, b, mat, gI, gE, L, R, mpad)
+        codes = np.concatenate([a, b]).astype(np.uint8)
+        s, ar, br = O.sw_batch_codes(codes, [0], [len(a)], [len(a)], [len(b)], mat, gI, gE, True)
+        assert h == int(s[0])
+        assert j == O.unpack_range(ar[0])[1] if h else True
+        assert i == O.unpack_range(br[0])[1] if h else True
+
+
+def test_model_matches_oracle_protein():
+    rng = np.random.default_rng(7)
+    mat = O.select_matrix(O.SIM_BLOSUM62)
+    for trial in range(20):
+        m, n = rng.integers(1, 40, 2)
+        a = rng.integers(0, 20, m)
+        b = rng.integers(0, 20, n)
+        if trial % 2:
+            b = np.concatenate([b[:3], a[2:], b[3:]])
+        h, i, j = model(a, b, mat, -10, -1, L=4, R=3, mpad=m + 3)
+        codes = np.concatenate([a, b]).astype(np.uint8)
+        s, ar, br = O.sw_batch_codes(codes, [0], [len(a)], [len(a)], [len(b)], mat, -10, -1, True)
+        assert (h, j, i) == (int(s[0]), O.unpack_range(ar[0])[1] if h else 0,
+                             O.unpack_range(br[0])[1] if h else 0)
+
+
+def test_model_flags_saturation():
+    # an exact score >= 32767 must surface as 32767 (the int32 re-run flag)
+    mat = np.full((25, 25), -1, np.int64)
+    np.fill_diagonal(mat, 127)
+    a = np.zeros(300, np.int64)  # 300 * 127 = 38100 > 32767
+    h, _, _ = model(a, a, mat, -10, -1, L=2, R=8)
+    assert h == SAT
