@@ -22,8 +22,10 @@ constexpr int kNssBins = 64;
 constexpr int kMBins = 256;
 constexpr int kMBinWidth = 16;
 constexpr int kNumBins = kNumClasses * kNssBins * kMBins;
-// Group-cost bins for the longest-first queue order.
-constexpr int kCostBins = 512;
+// Group-cost bins for the queue order: class-major, longest first in a class.
+constexpr int kCostBinsPerClass = 256;
+constexpr int kCostBins = 2048;  // >= kNumClasses * kCostBinsPerClass, multiple of 1024
+static_assert(kCostBins >= kNumClasses * kCostBinsPerClass, "cost bins");
 // Saturation value of the packed int16 path.
 constexpr int32_t kSat16 = 32767;
 
@@ -45,12 +47,14 @@ struct BucketCtl {
   int32_t error;       // bit 0: a length outside [0, kMaxSeqLen]; bit 1: an end
                        // position outside its sequence; bit 2: a bad group
   int32_t ovf_count;   // int16-saturated comparisons queued for int32
-  int32_t queue;       // persistent-kernel group counter
+  int32_t queue;       // unused (kept for layout)
   int32_t maxm;        // plain batching: longest A
   int32_t maxn;        // plain batching: longest B
   int32_t pad0;
   unsigned long long total_cells;
   unsigned long long padded_cells;
+  int32_t class_queue[kNumClasses];  // per-class persistent-kernel group counters
+  int32_t pad1[2];
 };
 
 // Arguments of the packed kernel.
@@ -63,16 +67,14 @@ struct FastArgs {
   const int16_t *table;        // KP x KP, row = A code, col = B code
   const int32_t *sorted;       // pair ids in bucket order (-1 = empty slot)
   const GroupDesc *groups;     // in processing order
-  const int32_t *ngroups;      // device scalar
-  int32_t *queue;              // device counter (zeroed before launch)
+  BucketCtl *ctl;              // group ranges per class, queue counters, error bits
   uint32_t *scratch;           // kWaveScratchWords per wave
   int32_t *scores;
   int32_t *a_range;
   int32_t *b_range;
   int32_t *ovf_list;           // pairs that saturated int16
   int32_t *ovf_count;
-  int32_t *error;              // bit 2: a group descriptor out of range
-  int32_t max_groups;          // bound of *ngroups
+  int32_t max_groups;          // bound of the group count
   uint32_t gap_o;              // -(gapInit + gapExtend), packed twice
   uint32_t gap_x;              // -gapExtend, packed twice
 };

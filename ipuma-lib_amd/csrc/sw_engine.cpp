@@ -206,10 +206,10 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   FastArgs A{};
   A.codes = codes; A.a_off = a_off; A.a_len = a_len; A.b_off = b_off; A.b_len = b_len;
   A.table = e->d_table; A.sorted = c.sorted.as<int32_t>(); A.groups = c.groups.as<GroupDesc>();
-  A.ngroups = &ctl->ngroups; A.queue = &ctl->queue; A.scratch = e->scratch.as<uint32_t>();
+  A.ctl = ctl; A.scratch = e->scratch.as<uint32_t>();
   A.scores = scores; A.a_range = a_range; A.b_range = b_range;
   A.ovf_list = c.ovf.as<int32_t>(); A.ovf_count = &ctl->ovf_count;
-  A.error = &ctl->error; A.max_groups = n + 64;
+  A.max_groups = n + 64;
   const uint32_t o = (uint32_t)(-(e->sw.gap_init + e->sw.gap_extend)) & 0xFFFFu;
   const uint32_t x = (uint32_t)(-e->sw.gap_extend) & 0xFFFFu;
   A.gap_o = o | (o << 16);

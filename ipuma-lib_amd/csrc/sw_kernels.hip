@@ -35,25 +35,20 @@ __device__ __forceinline__ uint32_t W(u16x2 v) { return __builtin_bit_cast(uint3
 __device__ __forceinline__ uint32_t W(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 
 // Value of lane k-1 of the same pipeline group (lanes k = lane % L); lanes
-// with k == 0 take `own`.  DPP row_shr:1 moves within rows of 16 lanes and
-// wave_shr:1 across the wave; with bound_ctrl off the first lane of the row /
-// wave keeps `own`.
+// with k == 0 take `own`.  Pipelines of up to 16 lanes move values with DPP
+// row_shr:1 inside rows of 16 lanes, the 64-lane pipeline with a shuffle;
+// the first lane of each pipeline always selects `own` explicitly.
 template <int L>
 __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool first) {
   if constexpr (L == 1) {
     (void)v; (void)first;
     return own;
   } else if constexpr (L == 64) {
-    (void)first;
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x138, 0xf, 0xf, false);
+    const uint32_t r = (uint32_t)__shfl_up((int)v, 1, 64);
+    return first ? own : r;
   } else {
-    uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x111, 0xf, 0xf, false);
-    if constexpr (L == 16) {
-      (void)first;
-      return r;
-    } else {
-      return first ? own : r;
-    }
+    const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x111, 0xf, 0xf, false);
+    return first ? own : r;
   }
 }
 
@@ -125,7 +120,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   for (int ss = 0; ss < nss; ++ss) {
     // The boundary row written by lane L-1 during the previous super-strip is
     // read back by lane 0: complete those stores and drop stale L1 lines.
-    if (ss > 0) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+    if (ss > 0) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     const int row0 = (ss * L + k) * R;
     uint32_t rp[R];
     {
@@ -282,48 +277,49 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   }
 }
 
-template <int KP>
+// One persistent kernel per pipeline class: its waves drain the class's range
+// of the group queue (longest first); a class with no groups exits at once.
+template <int L, int KP>
 __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
+  constexpr int CLS = L == 1 ? 0 : L == 2 ? 1 : L == 4 ? 2 : L == 8 ? 3 : L == 16 ? 4 : 5;
   __shared__ uint32_t tab[KP * KP * 32];
+  const int lane = threadIdx.x & 63;
+  const int g0 = max(A.ctl->group_base[CLS], 0);
+  const int g1 = min(A.ctl->group_base[CLS + 1], A.max_groups);
+  if (g1 <= g0) return;  // uniform over the grid
   for (int i = threadIdx.x; i < KP * KP * 32; i += blockDim.x)
     tab[i] = (uint32_t)(uint16_t)A.table[i >> 5];
   __syncthreads();
-  const int lane = threadIdx.x & 63;
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
-  const int ng = min(*A.ngroups, A.max_groups);
   for (;;) {
     int gi = 0;
-    if (lane == 0) gi = atomicAdd(A.queue, 1);
+    if (lane == 0) gi = g0 + atomicAdd(&A.ctl->class_queue[CLS], 1);
     gi = __builtin_amdgcn_readfirstlane(gi);
-    if (gi >= ng) break;
+    if (gi >= g1) break;
     const GroupDesc gd = A.groups[gi];
-    // a descriptor outside what the bucketing can produce (pipeline class,
-    // scratch row mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and
-    // skipped rather than run
-    bool ok = gd.cls >= 0 && gd.cls < kNumClasses && gd.base >= 0;
-    if (ok) {
-      const int L = kClassL[gd.cls];
-      ok = gd.mpad >= 0 && gd.mpad <= kMaxSeqLen && (int64_t)gd.mpad * (64 / L) <= 65536 &&
-           gd.nss >= 0 && (int64_t)(gd.nss - 1) * L * kRows < kMaxSeqLen;
-    }
+    // a descriptor outside what the bucketing produces (its class, scratch
+    // row mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and skipped
+    const bool ok = gd.cls == CLS && gd.base >= 0 && gd.mpad >= 0 && gd.mpad <= kMaxSeqLen &&
+                    (int64_t)gd.mpad * (64 / L) <= 65536 && gd.nss >= 0 &&
+                    (int64_t)(gd.nss - 1) * L * kRows < kMaxSeqLen;
     if (!ok) {
-      if (lane == 0) atomicOr(A.error, 4);
+      if (lane == 0) atomicOr(&A.ctl->error, 4);
       continue;
     }
-    switch (gd.cls) {
-      case 0: run_group<1, KP>(A, tab, gd, wscr, lane); break;
-      case 1: run_group<2, KP>(A, tab, gd, wscr, lane); break;
-      case 2: run_group<4, KP>(A, tab, gd, wscr, lane); break;
-      case 3: run_group<8, KP>(A, tab, gd, wscr, lane); break;
-      case 4: run_group<16, KP>(A, tab, gd, wscr, lane); break;
-      default: run_group<64, KP>(A, tab, gd, wscr, lane); break;
-    }
+    run_group<L, KP>(A, tab, gd, wscr, lane);
   }
 }
 
-template __global__ void k_fast<18>(FastArgs);
-template __global__ void k_fast<26>(FastArgs);
+#define IPUMA_FAST_INSTANCES(KP)                                  \
+  template __global__ void k_fast<1, KP>(FastArgs);               \
+  template __global__ void k_fast<2, KP>(FastArgs);               \
+  template __global__ void k_fast<4, KP>(FastArgs);               \
+  template __global__ void k_fast<8, KP>(FastArgs);               \
+  template __global__ void k_fast<16, KP>(FastArgs);              \
+  template __global__ void k_fast<64, KP>(FastArgs);
+IPUMA_FAST_INSTANCES(18)
+IPUMA_FAST_INSTANCES(26)
 
 // ---------------------------------------------------------------------------
 // Exact int32 kernel: the vertex loop of codelet_batch_affine.cpp:62-87 with
@@ -557,14 +553,14 @@ __global__ void k_scatter(int32_t n, const int32_t *bin_of, int32_t *offsets, in
 }
 
 __device__ __forceinline__ int cost_bin(int64_t cost) {
-  // 16 bins per octave, largest first after the descending scan
+  // 8 bins per octave of the group's step count
   if (cost <= 1) return 0;
   const float l2 = __log2f((float)cost);
-  return min(kCostBins - 1, (int)(l2 * 16.0f));
+  return min(kCostBinsPerClass - 1, (int)(l2 * 8.0f));
 }
 
 __global__ void k_groups(int32_t max_groups, const int32_t *sorted, const int32_t *a_len,
-                         const int32_t *b_len, const BucketCtl *ctl, GroupDesc *gtmp,
+                         const int32_t *b_len, BucketCtl *ctl, GroupDesc *gtmp,
                          int32_t *gbin, int32_t *costhist) {
   const int gi = blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= max_groups || gi >= ctl->ngroups) return;
@@ -587,28 +583,37 @@ __global__ void k_groups(int32_t max_groups, const int32_t *sorted, const int32_
   d.nss = (nbmax + L * kRows - 1) / (L * kRows);
   d.base = base;
   gtmp[gi] = d;
-  atomicAdd(&((BucketCtl *)ctl)->padded_cells,
-            (unsigned long long)128 * d.nss * kRows * (unsigned long long)mpad);
-  const int cb = cost_bin((int64_t)d.nss * (mpad + L - 1) * kRows);
+  atomicAdd(&ctl->padded_cells, (unsigned long long)128 * d.nss * kRows * (unsigned long long)mpad);
+  // bins run class by class, longest groups first inside a class
+  const int cb = c * kCostBinsPerClass + (kCostBinsPerClass - 1 -
+                                         cost_bin((int64_t)d.nss * (mpad + L - 1) * kRows));
   gbin[gi] = cb;
   atomicAdd(&costhist[cb], 1);
 }
 
-// Single block: descending exclusive scan of the cost bins.
-__global__ __launch_bounds__(kCostBins) void k_scan_cost(int32_t *costhist) {
-  __shared__ int32_t v[kCostBins];
+// Single block: exclusive scan of the class-major cost bins.
+__global__ __launch_bounds__(1024) void k_scan_cost(int32_t *costhist) {
+  __shared__ int32_t part[1024];
+  constexpr int PER = kCostBins / 1024;
+  static_assert(kCostBins % 1024 == 0, "cost bins");
   const int tid = threadIdx.x;
-  // position tid holds bin (kCostBins - 1 - tid): largest cost first
-  const int h = costhist[kCostBins - 1 - tid];
-  v[tid] = h;
+  int sum = 0;
+  for (int i = 0; i < PER; ++i) sum += costhist[tid * PER + i];
+  part[tid] = sum;
   __syncthreads();
-  for (int off = 1; off < kCostBins; off <<= 1) {
-    const int x = tid >= off ? v[tid - off] : 0;
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = tid >= off ? part[tid - off] : 0;
     __syncthreads();
-    v[tid] += x;
+    part[tid] += v;
     __syncthreads();
   }
-  costhist[kCostBins - 1 - tid] = v[tid] - h;
+  int run = part[tid] - sum;
+  for (int i = 0; i < PER; ++i) {
+    const int b = tid * PER + i;
+    const int h = costhist[b];
+    costhist[b] = run;
+    run += h;
+  }
 }
 
 __global__ void k_order_groups(int32_t max_groups, const BucketCtl *ctl, const GroupDesc *gtmp,
@@ -683,6 +688,7 @@ __global__ void k_plain_groups(int32_t n, BucketCtl *ctl, GroupDesc *groups) {
   const int gs = 128 / L;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = (n + gs - 1) / gs;
+  if (g <= kNumClasses) ctl->group_base[g] = g <= cls ? 0 : ng;
   if (g == 0) ctl->ngroups = ng;
   if (g >= ng) return;
   GroupDesc d;
@@ -724,8 +730,8 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
                      B.scores, B.a_range, B.b_range);
   if (B.plain) {
     hipLaunchKernelGGL(k_plain_setup, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.sorted);
-    hipLaunchKernelGGL(k_plain_groups, dim3(nblocks(n / 2 + 2, 256)), dim3(256), 0, st, n, B.ctl,
-                       B.groups);
+    hipLaunchKernelGGL(k_plain_groups, dim3(nblocks(n / 2 + kNumClasses + 2, 256)), dim3(256), 0,
+                       st, n, B.ctl, B.groups);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, B.hist, B.ctl);
@@ -733,25 +739,32 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
   const int gb = nblocks(B.max_groups, 256);
   hipLaunchKernelGGL(k_groups, dim3(gb), dim3(256), 0, st, B.max_groups, B.sorted, B.a_len,
                      B.b_len, B.ctl, B.gtmp, B.gbin, B.costhist);
-  hipLaunchKernelGGL(k_scan_cost, dim3(1), dim3(kCostBins), 0, st, B.costhist);
+  hipLaunchKernelGGL(k_scan_cost, dim3(1), dim3(1024), 0, st, B.costhist);
   hipLaunchKernelGGL(k_order_groups, dim3(gb), dim3(256), 0, st, B.max_groups, B.ctl, B.gtmp,
                      B.gbin, B.costhist, B.groups);
   return hipGetLastError();
 }
 
-hipError_t launch_fast(int kp, const FastArgs &A, int blocks, hipStream_t st) {
-  if (kp == 18)
-    hipLaunchKernelGGL(k_fast<18>, dim3(blocks), dim3(kFastThreads), 0, st, A);
-  else
-    hipLaunchKernelGGL(k_fast<26>, dim3(blocks), dim3(kFastThreads), 0, st, A);
+template <int KP>
+static hipError_t launch_fast_kp(const FastArgs &A, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_fast<1, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
+  hipLaunchKernelGGL((k_fast<2, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
+  hipLaunchKernelGGL((k_fast<4, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
+  hipLaunchKernelGGL((k_fast<8, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
+  hipLaunchKernelGGL((k_fast<16, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
+  hipLaunchKernelGGL((k_fast<64, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
   return hipGetLastError();
+}
+
+hipError_t launch_fast(int kp, const FastArgs &A, int blocks, hipStream_t st) {
+  return kp == 18 ? launch_fast_kp<18>(A, blocks, st) : launch_fast_kp<26>(A, blocks, st);
 }
 
 int fast_blocks_per_cu(int kp) {
   int nb = 0;
   hipError_t e = kp == 18
-      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<18>, kFastThreads, 0)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<26>, kFastThreads, 0);
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<1, 18>, kFastThreads, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<1, 26>, kFastThreads, 0);
   return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 

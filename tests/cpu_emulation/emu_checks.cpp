@@ -51,17 +51,21 @@ int main() {
     GroupDesc gd[2];
     gd[0] = {0, 8, 0x01020304, 0};  // nss from stray residue bytes
     gd[1] = {0, 8, 1, 0};           // the real group
-    int32_t ngroups = 2, queue = 0, error = 0, ovf = 0;
+    BucketCtl ctl;
+    std::memset(&ctl, 0, sizeof(ctl));
+    ctl.ngroups = 2;
+    for (int c = 1; c <= kNumClasses; ++c) ctl.group_base[c] = 2;  // both groups in class 0
+    int32_t ovf = 0;
     std::vector<uint32_t> scratch((size_t)kWaveScratchWords * (kFastThreads / 64), 0);
     int32_t score = -5, ar = 0, br = 0, ovf_list[4];
     FastArgs A{};
     A.codes = codes.data(); A.a_off = a_off; A.a_len = a_len; A.b_off = b_off; A.b_len = b_len;
-    A.table = table.data(); A.sorted = sorted.data(); A.groups = gd; A.ngroups = &ngroups;
-    A.queue = &queue; A.scratch = scratch.data(); A.scores = &score; A.a_range = &ar;
-    A.b_range = &br; A.ovf_list = ovf_list; A.ovf_count = &ovf; A.error = &error;
+    A.table = table.data(); A.sorted = sorted.data(); A.groups = gd; A.ctl = &ctl;
+    A.scratch = scratch.data(); A.scores = &score; A.a_range = &ar;
+    A.b_range = &br; A.ovf_list = ovf_list; A.ovf_count = &ovf;
     A.max_groups = 2; A.gap_o = 0x00010001u; A.gap_x = 0x00010001u;
     launch_fast(kp, A, 1, nullptr);
-    CHECK(error & 4, "corrupted descriptor raises error bit 4");
+    CHECK(ctl.error & 4, "corrupted descriptor raises error bit 4");
     CHECK(score == 8, "the valid group still runs (8 matches)");
   }
   std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);

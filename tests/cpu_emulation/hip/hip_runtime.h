@@ -138,6 +138,11 @@ void launch(const char *name, K kernel, dim3 grid, dim3 block, Args... args) {
 
 template <class T> inline T __shfl_xor(T v, int m) { return emu::exchange_get(v, emu::ctx.lane ^ m); }
 template <class T> inline T __shfl(T v, int src) { return emu::exchange_get(v, src); }
+template <class T> inline T __shfl_up(T v, unsigned d, int width = 64) {
+  const int lane = emu::ctx.lane;
+  const int src = (lane % width) >= (int)d ? lane - (int)d : lane;
+  return emu::exchange_get(v, src);
+}
 inline unsigned long long __ballot(int pred) {
   emu::Wave &w = *emu::ctx.wave;
   w.x[emu::ctx.lane] = pred ? 1 : 0;
