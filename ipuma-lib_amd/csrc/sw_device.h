@@ -22,14 +22,14 @@ constexpr int kNssBins = 64;
 constexpr int kMBins = 256;
 constexpr int kMBinWidth = 16;
 constexpr int kNumBins = kNumClasses * kNssBins * kMBins;
-// Group-cost bins for the queue order: class-major, longest first in a class.
+// Group-cost bins of the processing order: class-major, longest first in a class.
 constexpr int kCostBinsPerClass = 256;
 constexpr int kCostBins = 2048;  // >= kNumClasses * kCostBinsPerClass, multiple of 1024
 static_assert(kCostBins >= kNumClasses * kCostBinsPerClass, "cost bins");
 // Saturation value of the packed int16 path.
 constexpr int32_t kSat16 = 32767;
 
-// Group descriptor of the persistent queue.
+// Group descriptor: one wave's work in the packed kernel.
 struct GroupDesc {
   int32_t cls;   // class index into kClassL
   int32_t mpad;  // columns (|A|) padded for the group
@@ -46,16 +46,13 @@ struct BucketCtl {
   int32_t ngroups;
   int32_t error;       // bit 0: a length outside [0, kMaxSeqLen]; bit 1: an end
                        // position outside its sequence; bit 2: a bad group
-  int32_t ovf_count;   // int16-saturated comparisons queued for int32
-  int32_t queue;       // unused (kept for layout)
-  int32_t maxm;        // plain batching: longest A
-  int32_t maxn;        // plain batching: longest B
-  int32_t pad0;
+  int32_t ovf_count;   // int16-saturated comparisons listed for int32
+  int32_t maxm;        // longest A of the batch
+  int32_t maxn;        // longest B of the batch
   unsigned long long total_cells;
   unsigned long long padded_cells;
-  int32_t class_queue[kNumClasses];  // per-class persistent-kernel group counters
-  int32_t pad1[2];
 };
+static_assert(sizeof(BucketCtl) % 8 == 0, "BucketCtl is cleared as whole int32 words");
 
 // Arguments of the packed kernel.
 struct FastArgs {
@@ -67,7 +64,7 @@ struct FastArgs {
   const int16_t *table;        // KP x KP, row = A code, col = B code
   const int32_t *sorted;       // pair ids in bucket order (-1 = empty slot)
   const GroupDesc *groups;     // in processing order
-  BucketCtl *ctl;              // group ranges per class, queue counters, error bits
+  BucketCtl *ctl;              // group ranges per class, error bits
   uint32_t *scratch;           // kWaveScratchWords per wave
   int32_t *scores;
   int32_t *a_range;

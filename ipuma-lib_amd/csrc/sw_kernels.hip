@@ -10,7 +10,7 @@
 //   k_fast<KP>     packed int16, two comparisons per lane (low / high halves),
 //                  L lanes per comparison pipelined over the B rows (L = 1 is
 //                  the inter-sequence kernel, L = 64 the intra-sequence
-//                  anti-diagonal kernel), persistent over a queue of groups
+//                  anti-diagonal kernel), persistent over the list of groups
 //                  built by the device bucketing below.
 //   k_generic32    exact int32 re-statement of the vertex loop, one comparison
 //                  per lane; re-runs int16-saturated comparisons and serves
@@ -277,11 +277,21 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   }
 }
 
-// One persistent kernel per pipeline class: its waves drain the class's range
-// of the group queue (longest first); a class with no groups exits at once.
+// One persistent kernel per pipeline class: its waves walk the class's range
+// of the group list (longest first) in serpentine order -- round r gives wave
+// w the group r*W + w (r even) or r*W + W-1-w (r odd) -- so the first rounds
+// deal the longest groups out like a greedy longest-first schedule.  The walk
+// is a function of block and wave ids only: loop control stays in scalar
+// registers and every lane of a wave takes every iteration together, which
+// the cross-lane moves of run_group need.  (A queue fed by one lane's atomic
+// and read back with readfirstlane is not safe here: the compiler may thread
+// the first lane's result write at the end of one group into the next
+// iteration's fetch, after which the other lanes loop without it.)
+// A class with no groups exits at once.
 template <int L, int KP>
 __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   constexpr int CLS = L == 1 ? 0 : L == 2 ? 1 : L == 4 ? 2 : L == 8 ? 3 : L == 16 ? 4 : 5;
+  constexpr int WPB = kFastThreads / 64;
   __shared__ uint32_t tab[KP * KP * 32];
   const int lane = threadIdx.x & 63;
   const int g0 = max(A.ctl->group_base[CLS], 0);
@@ -290,24 +300,23 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   for (int i = threadIdx.x; i < KP * KP * 32; i += blockDim.x)
     tab[i] = (uint32_t)(uint16_t)A.table[i >> 5];
   __syncthreads();
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int wave = (int)blockIdx.x * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int nw = (int)gridDim.x * WPB;
   uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
-  for (;;) {
-    int gi = 0;
-    if (lane == 0) gi = g0 + atomicAdd(&A.ctl->class_queue[CLS], 1);
-    gi = __builtin_amdgcn_readfirstlane(gi);
-    if (gi >= g1) break;
+  int rev = 0;
+  for (int64_t rbase = g0; rbase < g1; rbase += nw, rev ^= 1) {
+    const int64_t gi = rbase + (rev ? nw - 1 - wave : wave);
+    if (gi >= g1) continue;
     const GroupDesc gd = A.groups[gi];
     // a descriptor outside what the bucketing produces (its class, scratch
     // row mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and skipped
     const bool ok = gd.cls == CLS && gd.base >= 0 && gd.mpad >= 0 && gd.mpad <= kMaxSeqLen &&
                     (int64_t)gd.mpad * (64 / L) <= 65536 && gd.nss >= 0 &&
                     (int64_t)(gd.nss - 1) * L * kRows < kMaxSeqLen;
-    if (!ok) {
-      if (lane == 0) atomicOr(&A.ctl->error, 4);
-      continue;
-    }
-    run_group<L, KP>(A, tab, gd, wscr, lane);
+    if (ok)
+      run_group<L, KP>(A, tab, gd, wscr, lane);
+    else if (lane == 0)
+      atomicOr(&A.ctl->error, 4);
   }
 }
 
@@ -383,12 +392,15 @@ __global__ __launch_bounds__(64) void k_generic32(GenericArgs G) {
     const int p = idx < total ? (G.list ? G.list[idx] : idx) : -1;
     const int m = p >= 0 ? G.a_len[p] : 0;
     if (p >= 0 && (m <= 1024 || m > kMaxSeqLen)) exact_pair(G, p, work + lane * 2048, work + lane * 2048 + 1024);
-    unsigned long long longs = __ballot(p >= 0 && m > 1024 && m <= kMaxSeqLen);
-    while (longs) {
-      const int l = __ffsll((long long)longs) - 1;
-      longs &= longs - 1;
-      const int pl = __shfl(p, l);
-      if (lane == 0) exact_pair(G, pl, work, work + 65536);
+    // Longer comparisons of this slice, one after another in lane 0 with the
+    // whole work area; lane 0 reads their ids itself, so no cross-lane
+    // operation depends on how the lanes reconverge after the loop above.
+    if (lane == 0) {
+      for (int l = 0; l < 64 && base + l < total; ++l) {
+        const int pl = G.list ? G.list[base + l] : base + l;
+        const int ml = G.a_len[pl];
+        if (ml > 1024 && ml <= kMaxSeqLen) exact_pair(G, pl, work, work + 65536);
+      }
     }
   }
 }
