@@ -9,7 +9,8 @@ constexpr int kRows = 16;
 // Threads per block of the packed kernel (one block per CU: the replicated
 // scoring table takes most of the LDS).
 constexpr int kFastThreads = 512;
-// Lanes-per-pair classes of the packed kernel (pipeline depth inside a wave).
+// Lanes-per-pair classes of the packed kernel (pipeline depth inside a wave);
+// class c < 5 has L = 2^c, the last class L = 64.
 constexpr int kNumClasses = 6;
 constexpr int kClassL[kNumClasses] = {1, 2, 4, 8, 16, 64};
 // Boundary scratch of one wave, in 32-bit words: mpad * (64/L) * 2 <= this.

@@ -105,6 +105,7 @@ struct ipuma_engine {
   int fast_blocks = 0;
   int generic_blocks = 512;
   int64_t target_slots = 0;
+  int rounds = 2;  // groups per resident wave the bucketing aims for
   DevBuf scratch, work;
   ChainBufs fwd, rev;
   DevBuf rcodes, rlen, rres;
@@ -151,7 +152,7 @@ int init_engine(ipuma_engine *e) {
   e->packed_ok = o >= 0 && x >= 0 && o <= 32767 && x <= 32767;
   e->forward_only = e->algo.forward_only ? 1 : 0;
   e->fast_blocks = e->num_cu * fast_blocks_per_cu(e->kp);
-  e->target_slots = (int64_t)e->fast_blocks * (kFastThreads / 64) * 128;
+  e->target_slots = (int64_t)e->fast_blocks * (kFastThreads / 64) * 128 * e->rounds;
   return IPUMA_OK;
 }
 
@@ -532,6 +533,14 @@ int ipuma_set_timing(ipuma_engine *e, int32_t on) {
   if (on && !e->ev[0])
     for (auto &ev : e->ev) HIP_TRY(hipEventCreate(&ev));
   e->timing = on ? 1 : 0;
+  return IPUMA_OK;
+}
+
+int ipuma_set_work_rounds(ipuma_engine *e, int32_t rounds) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  if (rounds < 1 || rounds > 64) return fail(IPUMA_EINVAL, "rounds must be in 1..64");
+  e->rounds = rounds;
+  e->target_slots = (int64_t)e->fast_blocks * (kFastThreads / 64) * 128 * e->rounds;
   return IPUMA_OK;
 }
 

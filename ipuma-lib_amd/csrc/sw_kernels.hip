@@ -34,16 +34,21 @@ __device__ __forceinline__ s16x2 S2(uint32_t v) { return __builtin_bit_cast(s16x
 __device__ __forceinline__ uint32_t W(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ uint32_t W(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 
-// Value of lane k-1 of the same pipeline group (lanes k = lane % L); lanes
-// with k == 0 take `own`.  Pipelines of up to 16 lanes move values with DPP
-// row_shr:1 inside rows of 16 lanes, the 64-lane pipeline with a shuffle;
-// the first lane of each pipeline always selects `own` explicitly.
-template <int L>
+// Pipeline modes of the packed kernel: one lane per comparison (L = 1, no
+// cross-lane moves), 2..16 lanes (DPP row_shr:1 inside rows of 16 lanes), and
+// the whole wave (L = 64, moves by shuffle).
+constexpr int kModeSolo = 0;
+constexpr int kModeRow = 1;
+constexpr int kModeWave = 2;
+
+// Value of lane k-1 of the same pipeline (k = lane % L); the first lane of a
+// pipeline (k == 0) takes `own`, selected explicitly.
+template <int MODE>
 __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool first) {
-  if constexpr (L == 1) {
+  if constexpr (MODE == kModeSolo) {
     (void)v; (void)first;
     return own;
-  } else if constexpr (L == 64) {
+  } else if constexpr (MODE == kModeWave) {
     const uint32_t r = (uint32_t)__shfl_up((int)v, 1, 64);
     return first ? own : r;
   } else {
@@ -60,8 +65,9 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
 // padded shape.  A super-strip is L*kRows rows of B; lane k of a pipeline owns
 // rows [ (ss*L + k)*kRows, +kRows ) and at step t computes column j = t - k of
 // them, receiving H/F of the row above (and the column's A code) from lane k-1
-// through DPP.  Lane 0 reads the row above the super-strip from the wave's
-// scratch, lane L-1 writes the last row back in place.
+// (from_prev).  Lane 0 reads the row above the super-strip from the wave's
+// scratch, lane L-1 writes the last row back in place.  L is a power of two
+// read from the group descriptor; the three modes compile the lane moves.
 //
 // Arithmetic is the reference recurrence with E and F clamped at 0:
 // H = max(diag + s, E', F') with E' = max(E, 0), F' = max(F, 0) is unchanged
@@ -79,16 +85,19 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
 // row maximum at its first column.  Rows merge in order with a strict >, lanes
 // merge at the end by (max H, min row).
 // ---------------------------------------------------------------------------
-template <int L, int KP>
+template <int MODE, int KP>
 __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__restrict__ tab,
                                           const GroupDesc gd, uint32_t *__restrict__ wscr,
-                                          int lane) {
+                                          int lane, int lg_rt) {
   constexpr int R = kRows;
-  constexpr int NG = 64 / L;
   constexpr uint32_t PADC = KP - 1;
   constexpr uint32_t COLSTRIDE = KP * 128;  // bytes of one A-code row of the LDS table
-  const int k = lane % L;
-  const int g = lane / L;
+  // pipeline depth L = 2^lg: fixed for the solo and whole-wave modes
+  const int lg = MODE == kModeSolo ? 0 : MODE == kModeWave ? 6 : lg_rt;
+  const int L = 1 << lg;
+  const int NG = 64 >> lg;  // pipelines per wave
+  const int k = lane & (L - 1);
+  const int g = lane >> lg;
   const bool first = (k == 0);
   const bool last = (k == L - 1);
   const int mpad = gd.mpad;
@@ -121,7 +130,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     // The boundary row written by lane L-1 during the previous super-strip is
     // read back by lane 0: complete those stores and drop stale L1 lines.
     if (ss > 0) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    const int row0 = (ss * L + k) * R;
+    const int row0 = ((ss << lg) + k) * R;
     uint32_t rp[R];
     {
       uint32_t wl[R / 4], wh[R / 4];
@@ -179,10 +188,10 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
           nb = make_uint2(0u, 0u);
         const uint32_t ac_lo = (t < m_lo) ? min((wa_lo >> (8 * u)) & 0xffu, PADC) : PADC;
         const uint32_t ac_hi = (t < m_hi) ? min((wa_hi >> (8 * u)) & 0xffu, PADC) : PADC;
-        const uint32_t rh = from_prev<L>(cb.x, out_h, first);
-        const uint32_t rf = from_prev<L>(cb.y, out_f, first);
-        col_lo = from_prev<L>(ac_lo * COLSTRIDE, col_lo, first);
-        col_hi = from_prev<L>(ac_hi * COLSTRIDE, col_hi, first);
+        const uint32_t rh = from_prev<MODE>(cb.x, out_h, first);
+        const uint32_t rf = from_prev<MODE>(cb.y, out_f, first);
+        col_lo = from_prev<MODE>(ac_lo * COLSTRIDE, col_lo, first);
+        col_hi = from_prev<MODE>(ac_hi * COLSTRIDE, col_hi, first);
         const int j = t - k;
         const uint32_t jinv = 0xFFFFu - (uint32_t)j;
         const char *tl = (const char *)tab + (col_lo + bank4);
@@ -238,8 +247,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     }
   }
 
-  if constexpr (L > 1) {
-#pragma unroll
+  if constexpr (MODE != kModeSolo) {
     for (int off = 1; off < L; off <<= 1) {
       const int oh = __shfl_xor(bh_lo, off), oi = __shfl_xor(bi_lo, off), oj = __shfl_xor(bj_lo, off);
       if (oh > bh_lo || (oh == bh_lo && oi < bi_lo)) {
@@ -277,26 +285,24 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   }
 }
 
-// One persistent kernel per pipeline class: its waves walk the class's range
-// of the group list (longest first) in serpentine order -- round r gives wave
+// One persistent kernel for the whole batch: its waves walk the group list
+// (longest first over all classes) in serpentine order -- round r gives wave
 // w the group r*W + w (r even) or r*W + W-1-w (r odd) -- so the first rounds
-// deal the longest groups out like a greedy longest-first schedule.  The walk
-// is a function of block and wave ids only: loop control stays in scalar
-// registers and every lane of a wave takes every iteration together, which
-// the cross-lane moves of run_group need.  (A queue fed by one lane's atomic
-// and read back with readfirstlane is not safe here: the compiler may thread
-// the first lane's result write at the end of one group into the next
-// iteration's fetch, after which the other lanes loop without it.)
-// A class with no groups exits at once.
-template <int L, int KP>
+// deal the longest groups out like a greedy longest-first schedule, and groups
+// of every pipeline depth run side by side.  The walk is a function of block
+// and wave ids only: loop control stays in scalar registers and every lane of
+// a wave takes every iteration together, which the cross-lane moves of
+// run_group need.  (A queue fed by one lane's atomic and read back with
+// readfirstlane is not safe here: the compiler may thread the first lane's
+// result write at the end of one group into the next iteration's fetch, after
+// which the other lanes loop without it.)
+template <int KP>
 __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
-  constexpr int CLS = L == 1 ? 0 : L == 2 ? 1 : L == 4 ? 2 : L == 8 ? 3 : L == 16 ? 4 : 5;
   constexpr int WPB = kFastThreads / 64;
   __shared__ uint32_t tab[KP * KP * 32];
   const int lane = threadIdx.x & 63;
-  const int g0 = max(A.ctl->group_base[CLS], 0);
-  const int g1 = min(A.ctl->group_base[CLS + 1], A.max_groups);
-  if (g1 <= g0) return;  // uniform over the grid
+  const int ng = min(A.ctl->ngroups, A.max_groups);
+  if (ng <= 0) return;  // uniform over the grid
   for (int i = threadIdx.x; i < KP * KP * 32; i += blockDim.x)
     tab[i] = (uint32_t)(uint16_t)A.table[i >> 5];
   __syncthreads();
@@ -304,31 +310,31 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   const int nw = (int)gridDim.x * WPB;
   uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
   int rev = 0;
-  for (int64_t rbase = g0; rbase < g1; rbase += nw, rev ^= 1) {
+  for (int64_t rbase = 0; rbase < ng; rbase += nw, rev ^= 1) {
     const int64_t gi = rbase + (rev ? nw - 1 - wave : wave);
-    if (gi >= g1) continue;
+    if (gi >= ng) continue;
     const GroupDesc gd = A.groups[gi];
-    // a descriptor outside what the bucketing produces (its class, scratch
-    // row mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and skipped
-    const bool ok = gd.cls == CLS && gd.base >= 0 && gd.mpad >= 0 && gd.mpad <= kMaxSeqLen &&
-                    (int64_t)gd.mpad * (64 / L) <= 65536 && gd.nss >= 0 &&
-                    (int64_t)(gd.nss - 1) * L * kRows < kMaxSeqLen;
-    if (ok)
-      run_group<L, KP>(A, tab, gd, wscr, lane);
-    else if (lane == 0)
-      atomicOr(&A.ctl->error, 4);
+    // a descriptor outside what the bucketing produces (class, scratch row
+    // mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and skipped
+    const int cls = gd.cls;
+    const int lg = cls < kNumClasses - 1 ? cls : 6;
+    const bool ok = cls >= 0 && cls < kNumClasses && gd.base >= 0 && gd.mpad >= 0 &&
+                    gd.mpad <= kMaxSeqLen && ((int64_t)gd.mpad << (6 - lg)) <= 65536 &&
+                    gd.nss >= 0 && ((int64_t)(gd.nss - 1) << lg) * kRows < kMaxSeqLen;
+    if (!ok) {
+      if (lane == 0) atomicOr(&A.ctl->error, 4);
+    } else if (cls == 0) {
+      run_group<kModeSolo, KP>(A, tab, gd, wscr, lane, 0);
+    } else if (cls == kNumClasses - 1) {
+      run_group<kModeWave, KP>(A, tab, gd, wscr, lane, 6);
+    } else {
+      run_group<kModeRow, KP>(A, tab, gd, wscr, lane, lg);
+    }
   }
 }
 
-#define IPUMA_FAST_INSTANCES(KP)                                  \
-  template __global__ void k_fast<1, KP>(FastArgs);               \
-  template __global__ void k_fast<2, KP>(FastArgs);               \
-  template __global__ void k_fast<4, KP>(FastArgs);               \
-  template __global__ void k_fast<8, KP>(FastArgs);               \
-  template __global__ void k_fast<16, KP>(FastArgs);              \
-  template __global__ void k_fast<64, KP>(FastArgs);
-IPUMA_FAST_INSTANCES(18)
-IPUMA_FAST_INSTANCES(26)
+template __global__ void k_fast<18>(FastArgs);
+template __global__ void k_fast<26>(FastArgs);
 
 // ---------------------------------------------------------------------------
 // Exact int32 kernel: the vertex loop of codelet_batch_affine.cpp:62-87 with
@@ -449,8 +455,14 @@ __device__ __forceinline__ int mbin_of(int m) {
   return m <= 2048 ? (m + 15) / 16 : min(kMBins - 1, 128 + (m - 2048 + 511) / 512);
 }
 
-__global__ void k_sum_cells(int32_t n, const int32_t *a_len, const int32_t *b_len, BucketCtl *ctl) {
+// Device-scope atomics on one address serialize across the XCDs, so the
+// batch-wide sums below are reduced per block first: one atomic per block.
+__global__ __launch_bounds__(256) void k_sum_cells(int32_t n, const int32_t *a_len,
+                                                   const int32_t *b_len, BucketCtl *ctl) {
+  __shared__ unsigned long long s_c[4];
+  __shared__ int s_m[4], s_n[4];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = threadIdx.x >> 6;
   unsigned long long c = 0;
   int m = 0, nb = 0;
   if (p < n) {
@@ -459,58 +471,75 @@ __global__ void k_sum_cells(int32_t n, const int32_t *a_len, const int32_t *b_le
     if (m > 0 && nb > 0 && m <= kMaxSeqLen && nb <= kMaxSeqLen) c = (unsigned long long)m * nb;
     else m = nb = 0;
   }
-  // wave reduction, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
     c += __shfl_xor(c, off);
     m = max(m, __shfl_xor(m, off));
     nb = max(nb, __shfl_xor(nb, off));
   }
   if ((threadIdx.x & 63) == 0) {
+    s_c[w] = c;
+    s_m[w] = m;
+    s_n[w] = nb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+      c += s_c[i];
+      m = max(m, s_m[i]);
+      nb = max(nb, s_n[i]);
+    }
     if (c) atomicAdd(&ctl->total_cells, c);
     atomicMax(&ctl->maxm, m);
     atomicMax(&ctl->maxn, nb);
   }
 }
 
-__global__ void k_classify(int32_t n, const int32_t *a_len, const int32_t *b_len,
-                           int64_t target_slots, int32_t force_class, int32_t *bin_of,
-                           int32_t *hist, BucketCtl *ctl, int32_t *scores, int32_t *a_range,
-                           int32_t *b_range) {
+__global__ __launch_bounds__(256) void k_classify(int32_t n, const int32_t *a_len,
+                                                  const int32_t *b_len, int64_t target_slots,
+                                                  int32_t force_class, int32_t *bin_of,
+                                                  int32_t *hist, BucketCtl *ctl, int32_t *scores,
+                                                  int32_t *a_range, int32_t *b_range) {
+  __shared__ int s_count[kNumClasses];
+  if (threadIdx.x < kNumClasses) s_count[threadIdx.x] = 0;
+  __syncthreads();
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int m = a_len[p], nb = b_len[p];
-  if (m < 0 || nb < 0 || m > kMaxSeqLen || nb > kMaxSeqLen) {
-    atomicOr(&ctl->error, 1);
-    bin_of[p] = -1;
-    scores[p] = -1;
-    a_range[p] = 0;
-    b_range[p] = 0;
-    return;
+  if (p < n) {
+    const int m = a_len[p], nb = b_len[p];
+    if (m < 0 || nb < 0 || m > kMaxSeqLen || nb > kMaxSeqLen) {
+      atomicOr(&ctl->error, 1);
+      bin_of[p] = -1;
+      scores[p] = -1;
+      a_range[p] = 0;
+      b_range[p] = 0;
+    } else if (m == 0 || nb == 0) {
+      bin_of[p] = -1;
+      scores[p] = 0;
+      a_range[p] = 0;
+      b_range[p] = 0;
+    } else {
+      int cls;
+      if (force_class >= 0) {
+        // never below the class the wave scratch needs (mpad * 64 / L <= 65536)
+        cls = max(force_class, pow2_class((m + 1023) / 1024));
+      } else {
+        const int64_t share =
+            max((int64_t)1, (int64_t)(ctl->total_cells / (unsigned long long)target_slots));
+        const int c_scr = pow2_class((m + 1023) / 1024);
+        const int c_share = pow2_class(((int64_t)m * nb + share - 1) / share);
+        const int c_rows = pow2_class((nb + kRows - 1) / kRows);
+        cls = max(c_scr, min(c_share, c_rows));
+      }
+      const int L = kClassL[cls];
+      const int nss = (nb + L * kRows - 1) / (L * kRows);
+      const int bin = (cls * kNssBins + min(nss, kNssBins - 1)) * kMBins + mbin_of(m);
+      bin_of[p] = bin;
+      atomicAdd(&hist[bin], 1);
+      atomicAdd(&s_count[cls], 1);
+    }
   }
-  if (m == 0 || nb == 0) {
-    bin_of[p] = -1;
-    scores[p] = 0;
-    a_range[p] = 0;
-    b_range[p] = 0;
-    return;
-  }
-  int cls;
-  if (force_class >= 0) {
-    // never below the class the wave scratch needs (mpad * 64 / L <= 65536)
-    cls = max(force_class, pow2_class((m + 1023) / 1024));
-  } else {
-    const int64_t share = max((int64_t)1, (int64_t)(ctl->total_cells / (unsigned long long)target_slots));
-    const int c_scr = pow2_class((m + 1023) / 1024);
-    const int c_share = pow2_class(((int64_t)m * nb + share - 1) / share);
-    const int c_rows = pow2_class((nb + kRows - 1) / kRows);
-    cls = max(c_scr, min(c_share, c_rows));
-  }
-  const int L = kClassL[cls];
-  const int nss = (nb + L * kRows - 1) / (L * kRows);
-  const int bin = (cls * kNssBins + min(nss, kNssBins - 1)) * kMBins + mbin_of(m);
-  bin_of[p] = bin;
-  atomicAdd(&hist[bin], 1);
-  atomicAdd(&ctl->class_count[cls], 1);
+  __syncthreads();
+  if (threadIdx.x < kNumClasses && s_count[threadIdx.x])
+    atomicAdd(&ctl->class_count[threadIdx.x], s_count[threadIdx.x]);
 }
 
 // Single block: exclusive scan of the bins, class-aligned starts, group counts.
@@ -596,9 +625,9 @@ __global__ void k_groups(int32_t max_groups, const int32_t *sorted, const int32_
   d.base = base;
   gtmp[gi] = d;
   atomicAdd(&ctl->padded_cells, (unsigned long long)128 * d.nss * kRows * (unsigned long long)mpad);
-  // bins run class by class, longest groups first inside a class
-  const int cb = c * kCostBinsPerClass + (kCostBinsPerClass - 1 -
-                                         cost_bin((int64_t)d.nss * (mpad + L - 1) * kRows));
+  // longest groups first over the whole batch (a wave's time is about its
+  // super-strips times its steps per super-strip)
+  const int cb = kCostBinsPerClass - 1 - cost_bin((int64_t)d.nss * (mpad + L - 1));
   gbin[gi] = cb;
   atomicAdd(&costhist[cb], 1);
 }
@@ -757,26 +786,19 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int KP>
-static hipError_t launch_fast_kp(const FastArgs &A, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_fast<1, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
-  hipLaunchKernelGGL((k_fast<2, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
-  hipLaunchKernelGGL((k_fast<4, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
-  hipLaunchKernelGGL((k_fast<8, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
-  hipLaunchKernelGGL((k_fast<16, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
-  hipLaunchKernelGGL((k_fast<64, KP>), dim3(blocks), dim3(kFastThreads), 0, st, A);
-  return hipGetLastError();
-}
-
 hipError_t launch_fast(int kp, const FastArgs &A, int blocks, hipStream_t st) {
-  return kp == 18 ? launch_fast_kp<18>(A, blocks, st) : launch_fast_kp<26>(A, blocks, st);
+  if (kp == 18)
+    hipLaunchKernelGGL(k_fast<18>, dim3(blocks), dim3(kFastThreads), 0, st, A);
+  else
+    hipLaunchKernelGGL(k_fast<26>, dim3(blocks), dim3(kFastThreads), 0, st, A);
+  return hipGetLastError();
 }
 
 int fast_blocks_per_cu(int kp) {
   int nb = 0;
   hipError_t e = kp == 18
-      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<1, 18>, kFastThreads, 0)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<1, 26>, kFastThreads, 0);
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<18>, kFastThreads, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<26>, kFastThreads, 0);
   return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 

@@ -128,6 +128,7 @@ def lib():
     L.ipuma_set_precision.argtypes = [_P, _i32]
     L.ipuma_set_forward_only.argtypes = [_P, _i32]
     L.ipuma_set_lanes_per_pair.argtypes = [_P, _i32]
+    L.ipuma_set_work_rounds.argtypes = [_P, _i32]
     L.ipuma_set_timing.argtypes = [_P, _i32]
     L.ipuma_encode_device.argtypes = [_P, _P, _i64, _P, _P]
     L.ipuma_last_batch_stats.argtypes = [_P, ctypes.POINTER(BatchStats)]
@@ -212,6 +213,10 @@ class SWAlgorithm:
 
     def set_lanes_per_pair(self, lanes: int):
         _check(lib().ipuma_set_lanes_per_pair(self._h, int(lanes)))
+
+    def set_work_rounds(self, rounds: int):
+        """Groups per resident wave the length bucketing aims for (1..64)."""
+        _check(lib().ipuma_set_work_rounds(self._h, int(rounds)))
 
     # --- reference surface --------------------------------------------------
     def _store(self, s, a, b):
