@@ -79,6 +79,17 @@ def cpu_baseline(batch, target_s):
                       f"({platform.processor() or platform.machine()})"}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the packed kernel on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a 1 GiB copy's counts)."""
+    p = os.path.join(ROOT, "profiles", "k_fast_protein_100k_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -175,6 +186,7 @@ def main():
         cpu = cpu_baseline(batch, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
 
+    tr = measured_traffic() if args.pairs == 100_000 else None
     out = {
         "metric": "GCUPS on 1 MI355X (bucketed vs plain, bit-exact); p50/p99 latency at fixed req/s",
         "value": round(gcups, 1),
@@ -198,7 +210,9 @@ def main():
         "parity_2000_pairs": parity,
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s (int16)", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                     "traffic": None,
+                     "traffic": tr["traffic_bytes"] if tr else None,
+                     "traffic_source": tr["source"] if tr else None,
+                     "hbm_gbs": (round(tr["traffic_bytes"] / (fast_avg * 1e6), 1) if tr else None),
                      "kernel": "k_fast<26>", "kernel_ms": round(fast_avg, 3),
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,

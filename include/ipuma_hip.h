@@ -154,7 +154,7 @@ int ipuma_set_forward_only(ipuma_engine *e, int32_t forward_only);
  * bucketing (default), or one of 1, 2, 4, 8, 16, 64 for every comparison.
  * No reference counterpart (the IPU runs one comparison per worker). */
 int ipuma_set_lanes_per_pair(ipuma_engine *e, int32_t lanes);
-/* Groups per resident wave the length bucketing aims for (1..64, default 2):
+/* Groups per resident wave the length bucketing aims for (1..64, default 1):
  * more rounds give the longest-first schedule finer pieces to balance, at the
  * price of deeper pipelines (more row padding) per comparison.  No reference
  * counterpart (the IPU's work split is fixed by its tile mapping). */

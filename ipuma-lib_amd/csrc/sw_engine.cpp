@@ -105,7 +105,7 @@ struct ipuma_engine {
   int fast_blocks = 0;
   int generic_blocks = 512;
   int64_t target_slots = 0;
-  int rounds = 2;  // groups per resident wave the bucketing aims for
+  int rounds = 1;  // groups per resident wave the bucketing aims for
   DevBuf scratch, work;
   ChainBufs fwd, rev;
   DevBuf rcodes, rlen, rres;
