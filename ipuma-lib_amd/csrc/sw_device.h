@@ -50,6 +50,8 @@ struct BucketCtl {
   int32_t ovf_count;   // int16-saturated comparisons listed for int32
   int32_t maxm;        // longest A of the batch
   int32_t maxn;        // longest B of the batch
+  int32_t queue;       // next group of the packed kernel's work queue
+  int32_t pad0;
   unsigned long long total_cells;
   unsigned long long padded_cells;
 };
@@ -103,7 +105,7 @@ struct BucketArgs {
   int32_t n;
   const int32_t *a_len;
   const int32_t *b_len;
-  int64_t target_slots;   // resident half-lanes: fair share = cells / this
+  int64_t target_slots;   // resident half-lanes x rounds: step cap = cells / (kRows * this)
   int32_t force_class;    // -1: choose per pair
   int32_t plain;          // 1: input order, L = 1, batch-max padding
   int32_t max_groups;

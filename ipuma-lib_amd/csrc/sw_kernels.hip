@@ -196,11 +196,14 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         const uint32_t jinv = 0xFFFFu - (uint32_t)j;
         const char *tl = (const char *)tab + (col_lo + bank4);
         const char *th = (const char *)tab + (col_hi + bank4);
+        // keep the row parts packed two per register across the loop (pinned
+        // once per step, ahead of the loads, so the 2*R loads can issue
+        // back to back and their latency overlaps)
+#pragma unroll
+        for (int r = 0; r < R; ++r) asm volatile("" : "+v"(rp[r]));
         uint32_t sv[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          // keep the row parts packed two per register across the loop
-          asm volatile("" : "+v"(rp[r]));
           const uint32_t slo = *(const uint16_t *)(tl + (rp[r] & 0xFFFFu));
           const uint32_t shi = *(const uint16_t *)(th + (rp[r] >> 16));
           sv[r] = __builtin_amdgcn_perm(shi, slo, 0x05040100u);
@@ -285,17 +288,21 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   }
 }
 
-// One persistent kernel for the whole batch: its waves walk the group list
-// (longest first over all classes) in serpentine order -- round r gives wave
-// w the group r*W + w (r even) or r*W + W-1-w (r odd) -- so the first rounds
-// deal the longest groups out like a greedy longest-first schedule, and groups
-// of every pipeline depth run side by side.  The walk is a function of block
-// and wave ids only: loop control stays in scalar registers and every lane of
-// a wave takes every iteration together, which the cross-lane moves of
-// run_group need.  (A queue fed by one lane's atomic and read back with
-// readfirstlane is not safe here: the compiler may thread the first lane's
-// result write at the end of one group into the next iteration's fetch, after
-// which the other lanes loop without it.)
+// Next entry of the group queue for the calling wave.  Every lane executes
+// the same atomic (lane 0 adds 1, the others 0), so no lane-conditional
+// branch surrounds it; the wave takes lane 0's ticket.  (A fetch written as
+// `if (lane == 0) atomicAdd` followed by readfirstlane is unsafe here: for
+// the 64-lane pipeline that branch has the same predicate as the first
+// lane's result write, the compiler may thread one into the other across the
+// loop back-edge, and lanes 1..63 then loop without lane 0.)
+__device__ __forceinline__ int next_group(int32_t *queue, int lane) {
+  const int t = atomicAdd(queue, lane == 0 ? 1 : 0);
+  return __builtin_amdgcn_readfirstlane(t);
+}
+
+// One persistent kernel for the whole batch: waves take groups from a queue
+// ordered longest first over all pipeline depths, so deep and shallow
+// pipelines run side by side and the tail is short.
 template <int KP>
 __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   constexpr int WPB = kFastThreads / 64;
@@ -307,12 +314,8 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
     tab[i] = (uint32_t)(uint16_t)A.table[i >> 5];
   __syncthreads();
   const int wave = (int)blockIdx.x * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int nw = (int)gridDim.x * WPB;
   uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
-  int rev = 0;
-  for (int64_t rbase = 0; rbase < ng; rbase += nw, rev ^= 1) {
-    const int64_t gi = rbase + (rev ? nw - 1 - wave : wave);
-    if (gi >= ng) continue;
+  for (int gi = next_group(&A.ctl->queue, lane); gi < ng; gi = next_group(&A.ctl->queue, lane)) {
     const GroupDesc gd = A.groups[gi];
     // a descriptor outside what the bucketing produces (class, scratch row
     // mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and skipped
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
                     gd.mpad <= kMaxSeqLen && ((int64_t)gd.mpad << (6 - lg)) <= 65536 &&
                     gd.nss >= 0 && ((int64_t)(gd.nss - 1) << lg) * kRows < kMaxSeqLen;
     if (!ok) {
-      if (lane == 0) atomicOr(&A.ctl->error, 4);
+      atomicOr(&A.ctl->error, 4);  // every lane: no lane-conditional branch here either
     } else if (cls == 0) {
       run_group<kModeSolo, KP>(A, tab, gd, wscr, lane, 0);
     } else if (cls == kNumClasses - 1) {
@@ -522,12 +525,19 @@ __global__ __launch_bounds__(256) void k_classify(int32_t n, const int32_t *a_le
         // never below the class the wave scratch needs (mpad * 64 / L <= 65536)
         cls = max(force_class, pow2_class((m + 1023) / 1024));
       } else {
-        const int64_t share =
-            max((int64_t)1, (int64_t)(ctl->total_cells / (unsigned long long)target_slots));
-        const int c_scr = pow2_class((m + 1023) / 1024);
-        const int c_share = pow2_class(((int64_t)m * nb + share - 1) / share);
-        const int c_rows = pow2_class((nb + kRows - 1) / kRows);
-        cls = max(c_scr, min(c_share, c_rows));
+        // The shallowest pipeline whose group takes at most `cap` steps, the
+        // batch's cells spread evenly over the resident half-lanes
+        // (cells / (kRows * slots)); the wave scratch sets a floor
+        // (mpad * 64/L <= 65536).  Groups of similar length then keep the
+        // longest-first queue's tail short.
+        const int64_t cap = max((int64_t)1, (int64_t)(ctl->total_cells /
+                                                      (unsigned long long)(kRows * target_slots)));
+        cls = pow2_class((m + 1023) / 1024);
+        while (cls < kNumClasses - 1) {
+          const int64_t Lc = kClassL[cls];
+          if ((int64_t)((nb + Lc * kRows - 1) / (Lc * kRows)) * (m + Lc - 1) <= cap) break;
+          ++cls;
+        }
       }
       const int L = kClassL[cls];
       const int nss = (nb + L * kRows - 1) / (L * kRows);
