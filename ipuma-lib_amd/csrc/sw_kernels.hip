@@ -82,8 +82,10 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
 // they came from: they never move the maximum or its first position.
 //
 // Position tracking: per row, key = H << 16 | (0xFFFF - j); the max key is the
-// row maximum at its first column.  Rows merge in order with a strict >, lanes
-// merge at the end by (max H, min row).
+// row maximum at its first column.  Pipelines of up to 16 lanes share one key
+// between rows 2q and 2q+1: H << 16 | (row 2q ? 1 : 0) << 15 | (0x7FFF - j),
+// which orders (max H, min row, min column) inside the pair.  Rows merge in
+// order with a strict >, lanes merge at the end by (max H, min row).
 // ---------------------------------------------------------------------------
 template <int MODE, int KP>
 __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__restrict__ tab,
@@ -94,6 +96,12 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   constexpr uint32_t COLSTRIDE = KP * 128;  // bytes of one A-code row of the LDS table
   // pipeline depth L = 2^lg: fixed for the solo and whole-wave modes
   const int lg = MODE == kModeSolo ? 0 : MODE == kModeWave ? 6 : lg_rt;
+  // Pipelines of up to 16 lanes have |A| <= 16384 (the wave scratch holds
+  // mpad * 64/L <= 65536 words), so 15 bits hold the column and one key
+  // covers a row pair; the whole-wave pipeline keeps one key per row with
+  // 16-bit columns (|A| up to 65535).
+  constexpr bool PAIRKEY = MODE != kModeWave;
+  constexpr int KN = PAIRKEY ? R / 2 : R;
   const int L = 1 << lg;
   const int NG = 64 >> lg;  // pipelines per wave
   const int k = lane & (L - 1);
@@ -150,13 +158,16 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         rp[r] = (cl * 128u) | ((ch * 128u) << 16);
       }
     }
-    uint32_t H[R], E[R], KL[R], KH[R];
+    uint32_t H[R], E[R], KL[KN], KH[KN];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       H[r] = 0u;
       E[r] = 0u;
-      KL[r] = 0u;
-      KH[r] = 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      KL[q] = 0u;
+      KH[q] = 0u;
     }
     uint32_t out_h = 0u, out_f = 0u, hdiag = 0u;
     uint32_t col_lo = PADC * COLSTRIDE, col_hi = PADC * COLSTRIDE;
@@ -193,7 +204,10 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         col_lo = from_prev<MODE>(ac_lo * COLSTRIDE, col_lo, first);
         col_hi = from_prev<MODE>(ac_hi * COLSTRIDE, col_hi, first);
         const int j = t - k;
-        const uint32_t jinv = 0xFFFFu - (uint32_t)j;
+        // key low parts: 16-bit 0xFFFF - j, or with row pairs the preferred
+        // (even) row's bit 15 over a 15-bit 0x7FFF - j
+        const uint32_t jinv = PAIRKEY ? ((0x7FFFu - (uint32_t)j) & 0x7FFFu) | 0x8000u
+                                      : 0xFFFFu - (uint32_t)j;
         const char *tl = (const char *)tab + (col_lo + bank4);
         const char *th = (const char *)tab + (col_hi + bank4);
         // keep the row parts packed two per register across the loop (pinned
@@ -213,6 +227,12 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         s16x2 t2 = __builtin_elementwise_add_sat(S2(hdiag), S2(sv[0]));
         hdiag = rh;
         uint32_t f = rf;
+        uint32_t kl_even = 0u, kh_even = 0u;
+        // the two row-pair key tails, computed once per step and pinned in
+        // VGPRs (otherwise they are re-derived per row, or a uniform tail
+        // lands in an SGPR where v_and_or_b32 cannot take it with its mask)
+        uint32_t jr_even = jinv, jr_odd = jinv & 0x7FFFu;
+        if constexpr (PAIRKEY) asm volatile("" : "+v"(jr_even), "+v"(jr_odd));
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           s16x2 t2n = t2;
@@ -223,8 +243,22 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
           const u16x2 ho = __builtin_elementwise_sub_sat(U2(hn), U2(gap_o));
           E[r] = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))));
           f = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(f), U2(gap_x))));
-          KL[r] = max(KL[r], __builtin_amdgcn_perm(hn, jinv, 0x05040100u));
-          KH[r] = max(KH[r], __builtin_amdgcn_perm(hn, jinv, 0x07060100u));
+          if constexpr (PAIRKEY) {
+            // H << 16 | row bit << 15 | 15-bit column; one max3 per row pair
+            const uint32_t jr = (r & 1) ? jr_odd : jr_even;
+            const uint32_t kl = (hn << 16) | jr;
+            const uint32_t kh = (hn & 0xFFFF0000u) | jr;
+            if (r & 1) {
+              KL[r >> 1] = max(max(KL[r >> 1], kl_even), kl);
+              KH[r >> 1] = max(max(KH[r >> 1], kh_even), kh);
+            } else {
+              kl_even = kl;
+              kh_even = kh;
+            }
+          } else {
+            KL[r] = max(KL[r], __builtin_amdgcn_perm(hn, jinv, 0x05040100u));
+            KH[r] = max(KH[r], __builtin_amdgcn_perm(hn, jinv, 0x07060100u));
+          }
           t2 = t2n;
         }
         out_h = H[R - 1];
@@ -233,19 +267,30 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
           *(uint2 *)(wscr + ((size_t)j * NG + g) * 2) = make_uint2(out_h, out_f);
       }
     }
+    // rows (or row pairs) in order, strict >: the first maximum in B order
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int hl = (int)(KL[r] >> 16);
+    for (int q = 0; q < KN; ++q) {
+      const int hl = (int)(KL[q] >> 16);
       if (hl > bh_lo) {
         bh_lo = hl;
-        bi_lo = row0 + r;
-        bj_lo = 0xFFFF - (int)(KL[r] & 0xFFFFu);
+        if constexpr (PAIRKEY) {
+          bi_lo = row0 + 2 * q + ((KL[q] & 0x8000u) ? 0 : 1);
+          bj_lo = 0x7FFF - (int)(KL[q] & 0x7FFFu);
+        } else {
+          bi_lo = row0 + q;
+          bj_lo = 0xFFFF - (int)(KL[q] & 0xFFFFu);
+        }
       }
-      const int hh = (int)(KH[r] >> 16);
+      const int hh = (int)(KH[q] >> 16);
       if (hh > bh_hi) {
         bh_hi = hh;
-        bi_hi = row0 + r;
-        bj_hi = 0xFFFF - (int)(KH[r] & 0xFFFFu);
+        if constexpr (PAIRKEY) {
+          bi_hi = row0 + 2 * q + ((KH[q] & 0x8000u) ? 0 : 1);
+          bj_hi = 0x7FFF - (int)(KH[q] & 0x7FFFu);
+        } else {
+          bi_hi = row0 + q;
+          bj_hi = 0xFFFF - (int)(KH[q] & 0xFFFFu);
+        }
       }
     }
   }

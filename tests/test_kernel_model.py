@@ -5,8 +5,9 @@ L*R rows, lane k computing column j = t - k at step t, H/F/code handed to lane
 k+1 each step, the row above each super-strip read from / written to a
 boundary row, E and F clamped at 0 with unsigned saturating subtraction, the
 substitution add saturating at 32767, pad code scores of -32768, per-row keys
-H << 16 | (0xFFFF - j), rows merged with a strict > and lanes by (max H, min
-row).  Agreement with the oracle on random inputs -- including pipeline fill
+H << 16 | (0xFFFF - j) -- or, for pipelines of up to 16 lanes, one key per
+row pair, H << 16 | (even row) << 15 | (0x7FFF - j) -- rows merged with a
+strict > and lanes by (max H, min row).  Agreement with the oracle on random inputs -- including pipeline fill
 and drain, padded rows and columns, and the saturation flag -- checks the
 algebra of the kernel before it ever runs on a device.
 """
@@ -18,7 +19,7 @@ import oracle as O
 SAT = 32767
 
 
-def model(a, b, mat, gI, gE, L, R, mpad=None):
+def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False):
     m, n = len(a), len(b)
     o, x = -(gI + gE), -gE
     dim = mat.shape[0]
@@ -40,6 +41,7 @@ def model(a, b, mat, gI, gE, L, R, mpad=None):
         H = np.zeros((L, R), np.int64)
         E = np.zeros((L, R), np.int64)
         key = np.zeros((L, R), np.int64)
+        pkey = np.zeros((L, (R + 1) // 2), np.int64)
         out_h = [0] * L
         out_f = [0] * L
         col = [pad] * L
@@ -60,6 +62,7 @@ def model(a, b, mat, gI, gE, L, R, mpad=None):
                     col[k] = prev_c[k - 1]
                 j = t - k
                 jinv = (0xFFFF - j) & 0xFFFF
+                j15 = (0x7FFF - j) & 0x7FFF
                 d = hdiag[k]
                 hdiag[k] = rh
                 f = rf
@@ -71,6 +74,8 @@ def model(a, b, mat, gI, gE, L, R, mpad=None):
                     E[k, r] = max(ho, subsat(E[k, r], x))
                     f = max(ho, subsat(f, x))
                     key[k, r] = max(key[k, r], (hn << 16) | jinv)
+                    pkey[k, r // 2] = max(pkey[k, r // 2],
+                                          (hn << 16) | ((r % 2 == 0) << 15) | j15)
                     d = H[k, r]
                     H[k, r] = hn
                 out_h[k] = int(H[k, R - 1])
@@ -80,6 +85,13 @@ def model(a, b, mat, gI, gE, L, R, mpad=None):
                     new_bf[j] = out_f[k]
         bnd_h, bnd_f = new_bh, new_bf
         for k in range(L):
+            if pairkey:
+                for q in range((R + 1) // 2):
+                    h = int(pkey[k, q] >> 16)
+                    if h > best[k][0]:
+                        r = 2 * q + (0 if pkey[k, q] & 0x8000 else 1)
+                        best[k] = (h, rows[k][r], 0x7FFF - int(pkey[k, q] & 0x7FFF))
+                continue
             for r in range(R):
                 h = int(key[k, r] >> 16)
                 if h > best[k][0]:
@@ -91,8 +103,10 @@ def model(a, b, mat, gI, gE, L, R, mpad=None):
     return h, i, j
 
 
-@pytest.mark.parametrize("L,R", [(1, 4), (2, 4), (4, 2), (3, 3), (8, 1)])
-def test_model_matches_oracle_dna(L, R):
+@pytest.mark.parametrize("L,R,pairkey", [(1, 4, False), (2, 4, False), (4, 2, False),
+                                          (3, 3, False), (8, 1, False), (1, 4, True),
+                                          (2, 6, True), (4, 2, True)])
+def test_model_matches_oracle_dna(L, R, pairkey):
     rng = np.random.default_rng(L * 10 + R)
     mat = O.select_matrix(O.SIM_NA, 1, -1, -1)
     for trial in range(25):
@@ -103,7 +117,7 @@ def test_model_matches_oracle_dna(L, R):
             b = np.concatenate([b[: n // 2], a, b[n // 2:]])
         gI, gE = [(0, -1), (-2, -1), (-4, -2)][trial % 3]
         mpad = m + int(rng.integers(0, 9))  # group padding
-        h, i, j = model(a, b, mat, gI, gE, L, R, mpad)
+        h, i, j = model(a, b, mat, gI, gE, L, R, mpad, pairkey)
         codes = np.concatenate([a, b]).astype(np.uint8)
         s, ar, br = O.sw_batch_codes(codes, [0], [len(a)], [len(a)], [len(b)], mat, gI, gE, True)
         assert h == int(s[0])
@@ -120,7 +134,7 @@ def test_model_matches_oracle_protein():
         b = rng.integers(0, 20, n)
         if trial % 2:
             b = np.concatenate([b[:3], a[2:], b[3:]])
-        h, i, j = model(a, b, mat, -10, -1, L=4, R=3, mpad=m + 3)
+        h, i, j = model(a, b, mat, -10, -1, L=4, R=3, mpad=m + 3, pairkey=bool(trial % 2))
         codes = np.concatenate([a, b]).astype(np.uint8)
         s, ar, br = O.sw_batch_codes(codes, [0], [len(a)], [len(a)], [len(b)], mat, -10, -1, True)
         assert (h, j, i) == (int(s[0]), O.unpack_range(ar[0])[1] if h else 0,
