@@ -8,7 +8,7 @@ namespace ipuma {
 constexpr int kRows = 16;
 // Threads per block of the packed kernel (one block per CU: the replicated
 // scoring table takes most of the LDS).
-constexpr int kFastThreads = 512;
+constexpr int kFastThreads = 768;
 // Lanes-per-pair classes of the packed kernel (pipeline depth inside a wave);
 // class c < 5 has L = 2^c, the last class L = 64.
 constexpr int kNumClasses = 6;
@@ -18,15 +18,20 @@ constexpr int kClassL[kNumClasses] = {1, 2, 4, 8, 16, 64};
 constexpr int64_t kWaveScratchWords = 65536 * 2;
 // Longest sequence the engine accepts (positions are packed as uint16).
 constexpr int32_t kMaxSeqLen = 65535;
-// Counting-sort bins of the length bucketing: class x super-strips x m-bin.
+// Counting-sort bins of the length bucketing: class x super-strip bin x m-bin.
+// A run is one (class, super-strip bin); groups never straddle runs, and each
+// run starts at a multiple of kRunAlign slots of the sorted pair array.
 constexpr int kNssBins = 64;
 constexpr int kMBins = 256;
 constexpr int kMBinWidth = 16;
-constexpr int kNumBins = kNumClasses * kNssBins * kMBins;
-// Group-cost bins of the processing order: class-major, longest first in a class.
+constexpr int kRuns = kNumClasses * kNssBins;
+constexpr int kNumBins = kRuns * kMBins;
+constexpr int kRunAlign = 128;
+constexpr int kSortedPad = kRuns * kRunAlign;  // empty slots the run alignment can add
+// Group-cost bins of the processing order (longest first).
 constexpr int kCostBinsPerClass = 256;
-constexpr int kCostBins = 2048;  // >= kNumClasses * kCostBinsPerClass, multiple of 1024
-static_assert(kCostBins >= kNumClasses * kCostBinsPerClass, "cost bins");
+constexpr int kCostBins = 2048;  // >= kCostBinsPerClass, multiple of 1024
+static_assert(kCostBins >= kCostBinsPerClass, "cost bins");
 // Saturation value of the packed int16 path.
 constexpr int32_t kSat16 = 32767;
 
@@ -41,9 +46,8 @@ struct GroupDesc {
 // Control block of the device bucketing (one per batch, zeroed before use).
 struct BucketCtl {
   int32_t class_count[kNumClasses];
-  int32_t class_start[kNumClasses];  // aligned start of the class in `sorted`
-  int32_t class_shift[kNumClasses];  // aligned start - unaligned prefix
-  int32_t group_base[kNumClasses + 1];
+  int32_t run_start[kRuns];           // aligned start of each run in `sorted`
+  int32_t run_group_base[kRuns + 1];  // first group of each run; [kRuns] = total
   int32_t ngroups;
   int32_t error;       // bit 0: a length outside [0, kMaxSeqLen]; bit 1: an end
                        // position outside its sequence; bit 2: a bad group
@@ -113,7 +117,7 @@ struct BucketArgs {
   int32_t *hist;          // kNumBins
   int32_t *costhist;      // kCostBins
   BucketCtl *ctl;
-  int32_t *sorted;        // n + 1024, pre-filled with -1
+  int32_t *sorted;        // n + kSortedPad, pre-filled with -1
   GroupDesc *gtmp;
   int32_t *gbin;
   GroupDesc *groups;

@@ -159,10 +159,10 @@ int init_engine(ipuma_engine *e) {
 int ensure_chain(ChainBufs &c, int32_t n) {
   int rc;
   const size_t nn = (size_t)std::max(n, 1);
-  const size_t max_groups = nn + 64;
+  const size_t max_groups = nn + kRuns;  // at most one part-filled group per run
   if ((rc = c.bin_of.ensure(nn * 4)) || (rc = c.hist.ensure((size_t)kNumBins * 4)) ||
       (rc = c.costhist.ensure((size_t)kCostBins * 4)) || (rc = c.ctl.ensure(sizeof(BucketCtl))) ||
-      (rc = c.sorted.ensure((nn + 1024) * 4)) || (rc = c.gtmp.ensure(max_groups * sizeof(GroupDesc))) ||
+      (rc = c.sorted.ensure((nn + kSortedPad) * 4)) || (rc = c.gtmp.ensure(max_groups * sizeof(GroupDesc))) ||
       (rc = c.gbin.ensure(max_groups * 4)) || (rc = c.groups.ensure(max_groups * sizeof(GroupDesc))) ||
       (rc = c.ovf.ensure(nn * 4)))
     return rc;
@@ -195,11 +195,11 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
   HIP_TRY(launch_fill_i32(c.hist.as<int32_t>(), kNumBins, 0, st));
   HIP_TRY(launch_fill_i32(c.costhist.as<int32_t>(), kCostBins, 0, st));
-  HIP_TRY(launch_fill_i32(c.sorted.as<int32_t>(), (int64_t)n + 1024, -1, st));
+  HIP_TRY(launch_fill_i32(c.sorted.as<int32_t>(), (int64_t)n + kSortedPad, -1, st));
   BucketArgs B{};
   B.n = n; B.a_len = a_len; B.b_len = b_len; B.target_slots = e->target_slots;
   B.force_class = e->force_class; B.plain = e->batching == IPUMA_BATCH_PLAIN ? 1 : 0;
-  B.max_groups = n + 64; B.bin_of = c.bin_of.as<int32_t>(); B.hist = c.hist.as<int32_t>();
+  B.max_groups = n + kRuns; B.bin_of = c.bin_of.as<int32_t>(); B.hist = c.hist.as<int32_t>();
   B.costhist = c.costhist.as<int32_t>(); B.ctl = ctl; B.sorted = c.sorted.as<int32_t>();
   B.gtmp = c.gtmp.as<GroupDesc>(); B.gbin = c.gbin.as<int32_t>(); B.groups = c.groups.as<GroupDesc>();
   B.scores = scores; B.a_range = a_range; B.b_range = b_range;
@@ -210,7 +210,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   A.ctl = ctl; A.scratch = e->scratch.as<uint32_t>();
   A.scores = scores; A.a_range = a_range; A.b_range = b_range;
   A.ovf_list = c.ovf.as<int32_t>(); A.ovf_count = &ctl->ovf_count;
-  A.max_groups = n + 64;
+  A.max_groups = n + kRuns;
   const uint32_t o = (uint32_t)(-(e->sw.gap_init + e->sw.gap_extend)) & 0xFFFFu;
   const uint32_t x = (uint32_t)(-e->sw.gap_extend) & 0xFFFFu;
   A.gap_o = o | (o << 16);

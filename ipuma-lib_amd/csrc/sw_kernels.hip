@@ -333,16 +333,16 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   }
 }
 
-// Next entry of the group queue for the calling wave.  Every lane executes
-// the same atomic (lane 0 adds 1, the others 0), so no lane-conditional
-// branch surrounds it; the wave takes lane 0's ticket.  (A fetch written as
-// `if (lane == 0) atomicAdd` followed by readfirstlane is unsafe here: for
-// the 64-lane pipeline that branch has the same predicate as the first
-// lane's result write, the compiler may thread one into the other across the
-// loop back-edge, and lanes 1..63 then loop without lane 0.)
-__device__ __forceinline__ int next_group(int32_t *queue, int lane) {
-  const int t = atomicAdd(queue, lane == 0 ? 1 : 0);
-  return __builtin_amdgcn_readfirstlane(t);
+// Next group for the calling wave, or -1 when the queue is empty.  Every
+// lane executes the same atomic (lane 0 adds 1, the others 0), so no
+// lane-conditional branch surrounds it; the wave takes lane 0's ticket.  (A
+// fetch written as `if (lane == 0) atomicAdd` followed by readfirstlane is
+// unsafe here: for the 64-lane pipeline that branch has the same predicate as
+// the first lane's result write, the compiler may thread one into the other
+// across the loop back-edge, and lanes 1..63 then loop without lane 0.)
+__device__ __forceinline__ int next_group(int32_t *queue, int ng, int lane) {
+  const int t = __builtin_amdgcn_readfirstlane(atomicAdd(queue, lane == 0 ? 1 : 0));
+  return t < ng ? t : -1;
 }
 
 // One persistent kernel for the whole batch: waves take groups from a queue
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   __syncthreads();
   const int wave = (int)blockIdx.x * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
-  for (int gi = next_group(&A.ctl->queue, lane); gi < ng; gi = next_group(&A.ctl->queue, lane)) {
+  for (int gi = next_group(&A.ctl->queue, ng, lane); gi >= 0; gi = next_group(&A.ctl->queue, ng, lane)) {
     const GroupDesc gd = A.groups[gi];
     // a descriptor outside what the bucketing produces (class, scratch row
     // mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and skipped
@@ -499,6 +499,13 @@ __device__ __forceinline__ int pow2_class(int64_t x) {
   return 5;
 }
 
+// Super-strip bin: exact below 32, then four bins per octave (a group pads
+// to its longest B, so neighbours in a bin differ by at most 19%).
+__device__ __forceinline__ int nss_bin_of(int nss) {
+  if (nss < 32) return nss;
+  return min(kNssBins - 1, 32 + (int)(4.0f * __log2f((float)nss * (1.0f / 32.0f))));
+}
+
 __device__ __forceinline__ int mbin_of(int m) {
   return m <= 2048 ? (m + 15) / 16 : min(kMBins - 1, 128 + (m - 2048 + 511) / 512);
 }
@@ -586,7 +593,7 @@ __global__ __launch_bounds__(256) void k_classify(int32_t n, const int32_t *a_le
       }
       const int L = kClassL[cls];
       const int nss = (nb + L * kRows - 1) / (L * kRows);
-      const int bin = (cls * kNssBins + min(nss, kNssBins - 1)) * kMBins + mbin_of(m);
+      const int bin = (cls * kNssBins + nss_bin_of(nss)) * kMBins + mbin_of(m);
       bin_of[p] = bin;
       atomicAdd(&hist[bin], 1);
       atomicAdd(&s_count[cls], 1);
@@ -597,46 +604,40 @@ __global__ __launch_bounds__(256) void k_classify(int32_t n, const int32_t *a_le
     atomicAdd(&ctl->class_count[threadIdx.x], s_count[threadIdx.x]);
 }
 
-// Single block: exclusive scan of the bins, class-aligned starts, group counts.
-__global__ __launch_bounds__(1024) void k_scan_bins(int32_t *hist, BucketCtl *ctl) {
-  __shared__ int32_t part[1024];
-  __shared__ int32_t cshift[kNumClasses];
-  constexpr int PER = kNumBins / 1024;
-  static_assert(kNumBins % 1024 == 0, "bins");
+// Single block: exclusive scan of the bins with every run (class, super-strip
+// bin) starting at a multiple of kRunAlign slots, and the first group of each
+// run.  Thread t owns the 128 bins [128 t, 128 t + 128): half of run t / 2.
+constexpr int kScanThreads = kNumBins / 128;
+static_assert(kNumBins % 128 == 0 && kMBins == 256 && kScanThreads <= 1024, "scan layout");
+__global__ __launch_bounds__(kScanThreads) void k_scan_bins(int32_t *hist, BucketCtl *ctl) {
+  __shared__ int32_t part[kScanThreads];
+  __shared__ int32_t rstart[kRuns];
   const int tid = threadIdx.x;
-  if (tid == 0) {
-    int unaligned = 0, aligned = 0, gb = 0;
-    for (int c = 0; c < kNumClasses; ++c) {
-      const int cnt = ctl->class_count[c];
-      const int gs = 128 / kClassL[c];
-      ctl->class_start[c] = aligned;
-      ctl->class_shift[c] = aligned - unaligned;
-      cshift[c] = aligned - unaligned;
-      ctl->group_base[c] = gb;
-      gb += (cnt + gs - 1) / gs;
-      unaligned += cnt;
-      aligned += (cnt + 127) / 128 * 128;
-    }
-    ctl->group_base[kNumClasses] = gb;
-    ctl->ngroups = gb;
-  }
   int sum = 0;
-  for (int i = 0; i < PER; ++i) sum += hist[tid * PER + i];
+  for (int i = 0; i < 128; ++i) sum += hist[tid * 128 + i];
   part[tid] = sum;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = tid >= off ? part[tid - off] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+  if (tid == 0) {
+    int aligned = 0, gb = 0;
+    for (int r = 0; r < kRuns; ++r) {
+      const int cnt = part[2 * r] + part[2 * r + 1];
+      const int gs = 128 / kClassL[r / kNssBins];
+      rstart[r] = aligned;
+      ctl->run_start[r] = aligned;
+      ctl->run_group_base[r] = gb;
+      gb += (cnt + gs - 1) / gs;
+      aligned += (cnt + kRunAlign - 1) / kRunAlign * kRunAlign;
+    }
+    ctl->run_group_base[kRuns] = gb;
+    ctl->ngroups = gb;
   }
-  int run = part[tid] - sum;
-  for (int i = 0; i < PER; ++i) {
-    const int b = tid * PER + i;
-    const int c = b / (kNssBins * kMBins);
+  __syncthreads();
+  int pos = rstart[tid >> 1] + ((tid & 1) ? part[tid - 1] : 0);
+  for (int i = 0; i < 128; ++i) {
+    const int b = tid * 128 + i;
     const int h = hist[b];
-    hist[b] = run + cshift[c];
-    run += h;
+    hist[b] = pos;
+    pos += h;
   }
 }
 
@@ -655,24 +656,37 @@ __device__ __forceinline__ int cost_bin(int64_t cost) {
   return min(kCostBinsPerClass - 1, (int)(l2 * 8.0f));
 }
 
-__global__ void k_groups(int32_t max_groups, const int32_t *sorted, const int32_t *a_len,
-                         const int32_t *b_len, BucketCtl *ctl, GroupDesc *gtmp,
-                         int32_t *gbin, int32_t *costhist) {
-  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per group: its run (binary search over the run group bases), its
+// slots, and the padded shape over the pairs it holds.
+__global__ __launch_bounds__(256) void k_groups(int32_t max_groups, const int32_t *sorted,
+                                                const int32_t *a_len, const int32_t *b_len,
+                                                BucketCtl *ctl, GroupDesc *gtmp, int32_t *gbin,
+                                                int32_t *costhist) {
+  const int gi = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
   if (gi >= max_groups || gi >= ctl->ngroups) return;
-  int c = 0;
-  while (c + 1 < kNumClasses && gi >= ctl->group_base[c + 1]) ++c;
+  int lo = 0, hi = kRuns - 1;  // last run whose first group is <= gi
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ctl->run_group_base[mid] <= gi) lo = mid; else hi = mid - 1;
+  }
+  const int c = lo / kNssBins;
   const int L = kClassL[c];
   const int gs = 128 / L;
-  const int base = ctl->class_start[c] + (gi - ctl->group_base[c]) * gs;
+  const int base = ctl->run_start[lo] + (gi - ctl->run_group_base[lo]) * gs;
   int mpad = 0, nbmax = 0;
-  for (int s = 0; s < gs; ++s) {
+  for (int s = lane; s < gs; s += 64) {
     const int p = sorted[base + s];
     if (p >= 0) {
       mpad = max(mpad, a_len[p]);
       nbmax = max(nbmax, b_len[p]);
     }
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    mpad = max(mpad, __shfl_xor(mpad, off));
+    nbmax = max(nbmax, __shfl_xor(nbmax, off));
+  }
+  if (lane != 0) return;
   GroupDesc d;
   d.cls = c;
   d.mpad = mpad;
@@ -687,7 +701,7 @@ __global__ void k_groups(int32_t max_groups, const int32_t *sorted, const int32_
   atomicAdd(&costhist[cb], 1);
 }
 
-// Single block: exclusive scan of the class-major cost bins.
+// Single block: exclusive scan of the cost bins.
 __global__ __launch_bounds__(1024) void k_scan_cost(int32_t *costhist) {
   __shared__ int32_t part[1024];
   constexpr int PER = kCostBins / 1024;
@@ -784,7 +798,6 @@ __global__ void k_plain_groups(int32_t n, BucketCtl *ctl, GroupDesc *groups) {
   const int gs = 128 / L;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = (n + gs - 1) / gs;
-  if (g <= kNumClasses) ctl->group_base[g] = g <= cls ? 0 : ng;
   if (g == 0) ctl->ngroups = ng;
   if (g >= ng) return;
   GroupDesc d;
@@ -830,11 +843,11 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
                        st, n, B.ctl, B.groups);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, st, B.hist, B.ctl);
+  hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(kScanThreads), 0, st, B.hist, B.ctl);
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.hist, B.sorted);
   const int gb = nblocks(B.max_groups, 256);
-  hipLaunchKernelGGL(k_groups, dim3(gb), dim3(256), 0, st, B.max_groups, B.sorted, B.a_len,
-                     B.b_len, B.ctl, B.gtmp, B.gbin, B.costhist);
+  hipLaunchKernelGGL(k_groups, dim3(nblocks((int64_t)B.max_groups * 64, 256)), dim3(256), 0, st,
+                     B.max_groups, B.sorted, B.a_len, B.b_len, B.ctl, B.gtmp, B.gbin, B.costhist);
   hipLaunchKernelGGL(k_scan_cost, dim3(1), dim3(1024), 0, st, B.costhist);
   hipLaunchKernelGGL(k_order_groups, dim3(gb), dim3(256), 0, st, B.max_groups, B.ctl, B.gtmp,
                      B.gbin, B.costhist, B.groups);

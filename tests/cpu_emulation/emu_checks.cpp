@@ -54,7 +54,6 @@ int main() {
     BucketCtl ctl;
     std::memset(&ctl, 0, sizeof(ctl));
     ctl.ngroups = 2;
-    for (int c = 1; c <= kNumClasses; ++c) ctl.group_base[c] = 2;  // both groups in class 0
     int32_t ovf = 0;
     std::vector<uint32_t> scratch((size_t)kWaveScratchWords * (kFastThreads / 64), 0);
     int32_t score = -5, ar = 0, br = 0, ovf_list[4];
