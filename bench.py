@@ -1,13 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the batched Smith-Waterman engine on one MI355X.
 
-Workload (default): the protein batch -- 100,000 BLOSUM62 pairs, lognormal
-lengths (median 300, sigma 0.5, clipped to 50..1024), half of them related at
-40% identity, gap open 11 / extend 1 -- synthetic and seeded, resident in HBM
-before the timed region.  One step = one device batch over all pairs:
-device-side bucketing, the packed int16 kernel, the exact int32 re-run of
-saturated pairs; scores plus A/B end positions (forward pass, as the
-reference's benchmark runs with --forwardOnly).
+Workloads (synthetic, seeded, resident in HBM before the timed region):
+  protein (default, the headline): 100,000 BLOSUM62 pairs, lognormal lengths
+    (median 300, sigma 0.5, clipped to 50..1024), half of them related at 40%
+    identity, gap open 11 / extend 1; compared with the plain kernel padded to
+    the batch maximum.
+  skewed: 1,000,000 DNA pairs of 150 x U[150,400] (B embeds a mutated A) plus
+    5,000 pairs with power-law lengths (alpha 1.5) over 1-16 kb at 85%
+    identity, open 3 / extend 1; compared with the plain kernel that pads each
+    wave's group of input-order pairs to its own longest pair (padding every
+    group to the batch maximum would take minutes per step here).
+One step = one device batch over all pairs: device-side bucketing, the packed
+int16 kernel, the exact int32 re-run of saturated pairs; scores plus A/B end
+positions (forward pass, as the reference's benchmark runs with --forwardOnly).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -43,6 +49,9 @@ def parse():
     ap.add_argument("--with-starts", action="store_true", help="time the reverse pass too")
     ap.add_argument("--rounds", type=int, default=0,
                     help="groups per resident wave for the bucketing (0 = engine default)")
+    ap.add_argument("--config", choices=["protein", "skewed"], default="protein",
+                    help="protein: 100k BLOSUM62 pairs (the headline); skewed: 1M short + 5k "
+                         "long DNA pairs, bucketed vs the plain padded batch")
     return ap.parse_args()
 
 
@@ -53,7 +62,9 @@ def cpu_baseline(batch, target_s):
     import oracle as O
     O.build()
     sw = batch["sw"]
-    mat = O.select_matrix(O.SIM_BLOSUM62)
+    mat = (O.select_matrix(O.SIM_BLOSUM62) if sw["similarity"] == "blosum62"
+           else O.select_matrix(O.SIM_NA, sw["matchValue"], sw["mismatchValue"],
+                                sw["ambiguityValue"]))
     threads = O.max_threads()
     cells = batch["a_len"].astype(np.int64) * batch["b_len"].astype(np.int64)
     csum = np.cumsum(cells)
@@ -103,7 +114,21 @@ def main():
     from ipuma_amd import workloads as W
 
     t = time.time()
-    batch = W.config1(n=args.pairs)
+    if args.config == "protein":
+        batch = W.config1(n=args.pairs)
+        plain_mode, plain_name = I.BATCH_PLAIN, "plain kernel padded to the batch maximum"
+        workload = ("protein batch: 100k BLOSUM62 pairs, lognormal(300, 0.5) lengths clipped "
+                    "50-1024, half related at 40% identity, gap open 11 extend 1, scores + end "
+                    "positions")
+        data = "synthetic (seeded generator, BLOSUM62 background residues)"
+    else:
+        batch = W.config2()
+        plain_mode, plain_name = (I.BATCH_PLAIN_TILE,
+                                  "plain kernel, input order, each wave padded to its longest pair")
+        workload = ("skewed DNA batch: 1,000,000 pairs 150 x U[150,400] (B embeds a mutated A) "
+                    "+ 5,000 pairs power-law(alpha 1.5) 1-16 kb at 85% identity, match 1 "
+                    "mismatch -1 gap open 3 extend 1, scores + end positions")
+        data = "synthetic (seeded generator, uniform ACGT)"
     log(f"generated {len(batch['a_len'])} pairs, {batch['cells']:.3e} cells in {time.time()-t:.1f}s")
     dev = torch.device("cuda:0")
     codes = torch.from_numpy(batch["codes"]).to(dev)
@@ -157,16 +182,20 @@ def main():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     k = min(2000, n)
+    sw = batch["sw"]
+    mat = (O.select_matrix(O.SIM_BLOSUM62) if sw["similarity"] == "blosum62"
+           else O.select_matrix(O.SIM_NA, sw["matchValue"], sw["mismatchValue"],
+                                sw["ambiguityValue"]))
     exp = O.sw_batch_codes(batch["codes"], batch["a_off"][:k], batch["a_len"][:k],
-                           batch["b_off"][:k], batch["b_len"][:k], O.select_matrix(O.SIM_BLOSUM62),
-                           batch["sw"]["gapInit"], batch["sw"]["gapExtend"], not args.with_starts)
+                           batch["b_off"][:k], batch["b_len"][:k], mat,
+                           sw["gapInit"], sw["gapExtend"], not args.with_starts)
     got = (scores[:k].cpu().numpy(), ar[:k].cpu().numpy(), br[:k].cpu().numpy())
     parity = all(bool((g == e).all()) for g, e in zip(got, exp))
     log(f"parity on {k} pairs: {parity}")
 
     plain = None
     if args.plain_steps > 0:
-        eng.set_batching(I.BATCH_PLAIN)
+        eng.set_batching(plain_mode)
         step()
         torch.cuda.synchronize()
         tp = time.perf_counter()
@@ -175,8 +204,8 @@ def main():
         torch.cuda.synchronize()
         pe = (time.perf_counter() - tp) / args.plain_steps
         pst = eng.stats()
-        plain = {"ms_per_step": round(pe * 1e3, 2), "gcups": round(cells / pe / 1e9, 1),
-                 "padded_cells": pst["padded_cells"]}
+        plain = {"kind": plain_name, "ms_per_step": round(pe * 1e3, 2),
+                 "gcups": round(cells / pe / 1e9, 1), "padded_cells": pst["padded_cells"]}
         log(f"plain padded batch: {plain}")
         eng.set_batching(I.BATCH_BUCKETED)
 
@@ -186,7 +215,7 @@ def main():
         cpu = cpu_baseline(batch, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
 
-    tr = measured_traffic() if args.pairs == 100_000 else None
+    tr = measured_traffic() if (args.config == "protein" and args.pairs == 100_000) else None
     out = {
         "metric": "GCUPS on 1 MI355X (bucketed vs plain, bit-exact); p50/p99 latency at fixed req/s",
         "value": round(gcups, 1),
@@ -199,10 +228,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int16x2 (int32 re-run on saturation)",
-        "data": "synthetic (seeded generator, BLOSUM62 background residues)",
-        "config": {"workload": "protein batch: 100k BLOSUM62 pairs, lognormal(300, 0.5) "
-                               "lengths clipped 50-1024, half related at 40% identity, "
-                               "gap open 11 extend 1, scores + end positions",
+        "data": data,
+        "config": {"workload": workload,
                    "pairs": n, "cells": cells,
                    "starts": bool(args.with_starts), "work_rounds": args.rounds or "default"},
         "bucketed_vs_plain": (round(gcups / plain["gcups"], 2) if plain else None),
@@ -213,7 +240,8 @@ def main():
                      "traffic": tr["traffic_bytes"] if tr else None,
                      "traffic_source": tr["source"] if tr else None,
                      "hbm_gbs": (round(tr["traffic_bytes"] / (fast_avg * 1e6), 1) if tr else None),
-                     "kernel": "k_fast<26>", "kernel_ms": round(fast_avg, 3),
+                     "kernel": f"k_fast<{26 if args.config == 'protein' else 18}>",
+                     "kernel_ms": round(fast_avg, 3),
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,
     }

@@ -139,8 +139,10 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
 /* Batching policy of the device path (replaces IPUAlgoConfig.fillAlgo's
  * partition::fillFirst/roundRobin/greedy, src/partition.cpp:197-462). */
 enum {
-  IPUMA_BATCH_BUCKETED = 0, /* device length bucketing + persistent queue (default) */
-  IPUMA_BATCH_PLAIN = 1     /* one pair per lane, natural order, padded to batch max */
+  IPUMA_BATCH_BUCKETED = 0,  /* device length bucketing + persistent queue (default) */
+  IPUMA_BATCH_PLAIN = 1,     /* one pair per lane, input order, padded to the batch max */
+  IPUMA_BATCH_PLAIN_TILE = 2 /* one pair per lane, input order, each wave's group padded
+                                to its own longest pair */
 };
 /* Precision policy of the device path. */
 enum {

@@ -111,7 +111,8 @@ struct BucketArgs {
   const int32_t *b_len;
   int64_t target_slots;   // resident half-lanes x rounds: step cap = cells / (kRows * this)
   int32_t force_class;    // -1: choose per pair
-  int32_t plain;          // 1: input order, L = 1, batch-max padding
+  int32_t plain;          // 0: bucketed; 1: input order, batch-max padding; 2: input order,
+                          // each group padded to its own longest pair
   int32_t max_groups;
   int32_t *bin_of;
   int32_t *hist;          // kNumBins

@@ -198,7 +198,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   HIP_TRY(launch_fill_i32(c.sorted.as<int32_t>(), (int64_t)n + kSortedPad, -1, st));
   BucketArgs B{};
   B.n = n; B.a_len = a_len; B.b_len = b_len; B.target_slots = e->target_slots;
-  B.force_class = e->force_class; B.plain = e->batching == IPUMA_BATCH_PLAIN ? 1 : 0;
+  B.force_class = e->force_class; B.plain = e->batching == IPUMA_BATCH_PLAIN ? 1 : e->batching == IPUMA_BATCH_PLAIN_TILE ? 2 : 0;
   B.max_groups = n + kRuns; B.bin_of = c.bin_of.as<int32_t>(); B.hist = c.hist.as<int32_t>();
   B.costhist = c.costhist.as<int32_t>(); B.ctl = ctl; B.sorted = c.sorted.as<int32_t>();
   B.gtmp = c.gtmp.as<GroupDesc>(); B.gbin = c.gbin.as<int32_t>(); B.groups = c.groups.as<GroupDesc>();
@@ -509,7 +509,8 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
 }
 
 int ipuma_set_batching(ipuma_engine *e, int32_t mode) {
-  if (!e || (mode != IPUMA_BATCH_BUCKETED && mode != IPUMA_BATCH_PLAIN))
+  if (!e || (mode != IPUMA_BATCH_BUCKETED && mode != IPUMA_BATCH_PLAIN &&
+             mode != IPUMA_BATCH_PLAIN_TILE))
     return fail(IPUMA_EINVAL, "invalid batching mode");
   e->batching = mode;
   return IPUMA_OK;

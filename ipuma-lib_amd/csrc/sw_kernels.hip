@@ -781,8 +781,11 @@ __global__ void k_finish_starts(int32_t n, const int32_t *scores, const int32_t 
 }
 
 // ---------------------------------------------------------------------------
-// Plain padded batch: pairs in input order, one pair per lane (L = 1), every
-// group padded to the longest A and B of the whole batch.
+// Plain padded batches (the baselines the bucketing is measured against):
+// pairs in input order, one pair per lane unless the longest A of the batch
+// needs a deeper pipeline for the wave scratch (|A| > 1024); group g holds
+// pairs [g*gs, (g+1)*gs).  Mode 1 pads every group to the longest A and B of
+// the whole batch, mode 2 pads each group to its own longest pair.
 // ---------------------------------------------------------------------------
 __global__ void k_plain_setup(int32_t n, const int32_t *bin_of, int32_t *sorted) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -790,20 +793,42 @@ __global__ void k_plain_setup(int32_t n, const int32_t *bin_of, int32_t *sorted)
   sorted[i] = bin_of[i] >= 0 ? i : -1;
 }
 
-// One pair per lane unless the longest A needs a deeper pipeline for the
-// wave scratch (|A| > 1024); group g holds pairs [g*gs, (g+1)*gs) in order.
-__global__ void k_plain_groups(int32_t n, BucketCtl *ctl, GroupDesc *groups) {
+// One wave per group.
+__global__ __launch_bounds__(256) void k_plain_groups(int32_t n, int32_t per_group,
+                                                      const int32_t *sorted,
+                                                      const int32_t *a_len,
+                                                      const int32_t *b_len, BucketCtl *ctl,
+                                                      GroupDesc *groups) {
   const int cls = pow2_class((ctl->maxm + 1023) / 1024);
   const int L = kClassL[cls];
   const int gs = 128 / L;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
   const int ng = (n + gs - 1) / gs;
-  if (g == 0) ctl->ngroups = ng;
+  if (g == 0 && lane == 0) ctl->ngroups = ng;
   if (g >= ng) return;
+  int mpad = ctl->maxm, nbmax = ctl->maxn;
+  if (per_group) {
+    mpad = 0;
+    nbmax = 0;
+    for (int s = lane; s < gs; s += 64) {
+      const int i = g * gs + s;
+      const int p = i < n ? sorted[i] : -1;
+      if (p >= 0) {
+        mpad = max(mpad, a_len[p]);
+        nbmax = max(nbmax, b_len[p]);
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      mpad = max(mpad, __shfl_xor(mpad, off));
+      nbmax = max(nbmax, __shfl_xor(nbmax, off));
+    }
+  }
+  if (lane != 0) return;
   GroupDesc d;
   d.cls = cls;
-  d.mpad = ctl->maxm;
-  d.nss = (ctl->maxn + L * kRows - 1) / (L * kRows);
+  d.mpad = mpad;
+  d.nss = (nbmax + L * kRows - 1) / (L * kRows);
   d.base = g * gs;
   groups[g] = d;
   atomicAdd(&ctl->padded_cells,
@@ -839,8 +864,8 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
                      B.scores, B.a_range, B.b_range);
   if (B.plain) {
     hipLaunchKernelGGL(k_plain_setup, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.sorted);
-    hipLaunchKernelGGL(k_plain_groups, dim3(nblocks(n / 2 + kNumClasses + 2, 256)), dim3(256), 0,
-                       st, n, B.ctl, B.groups);
+    hipLaunchKernelGGL(k_plain_groups, dim3(nblocks((int64_t)n * 64 + 64, 256)), dim3(256), 0,
+                       st, n, B.plain == 2 ? 1 : 0, B.sorted, B.a_len, B.b_len, B.ctl, B.groups);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(kScanThreads), 0, st, B.hist, B.ctl);

@@ -30,7 +30,7 @@ SIMILARITY = {"na": 0, "blosum50": 1, "blosum62": 2, "simple": 3}
 DATATYPE = {"na": 0, "aa": 1, "str": 2}
 VERTEX_TYPE = {"cpp": 0, "assembly": 1, "multi": 2, "multiasm": 3}
 FILL_ALGO = {"fillfirst": 0, "roundrobin": 1, "greedy": 2}
-BATCH_BUCKETED, BATCH_PLAIN = 0, 1
+BATCH_BUCKETED, BATCH_PLAIN, BATCH_PLAIN_TILE = 0, 1, 2
 PREC_AUTO, PREC_INT32 = 0, 1
 
 

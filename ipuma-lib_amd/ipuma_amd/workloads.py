@@ -204,19 +204,100 @@ def _powerlaw_len(rng, k, alpha, lo, hi):
     return np.round(x).astype(np.int64)
 
 
+def arena_from_flat(seqs_flat: np.ndarray, lens: np.ndarray):
+    """Pack sequences given back to back in `seqs_flat` (lengths `lens`) into an
+    arena with 4-byte aligned offsets; returns (codes, offsets)."""
+    lens = np.asarray(lens, np.int64)
+    padded = (lens + 3) // 4 * 4
+    offs = np.zeros(len(lens), np.int64)
+    if len(lens) > 1:
+        offs[1:] = np.cumsum(padded)[:-1]
+    total = int(padded.sum()) + 16
+    codes = np.zeros(total, np.uint8)
+    src_start = np.zeros(len(lens), np.int64)
+    if len(lens) > 1:
+        src_start[1:] = np.cumsum(lens)[:-1]
+    # destination index of every residue
+    rep = np.repeat(offs - src_start, lens)
+    codes[np.arange(len(seqs_flat), dtype=np.int64) + rep] = seqs_flat
+    return codes, offs
+
+
+def _embedded_pairs_vec(rng, n, la, lb_lo, lb_hi, p_sub, p_indel):
+    """Vectorised dna_embedded_pair for n pairs with |A| = la, |B| ~ U[lb_lo, lb_hi]."""
+    A = DNA_CODES[rng.integers(0, 4, (n, la))]
+    u = rng.random((n, la))
+    indel = u < p_indel
+    sub = (~indel) & (u < p_indel + p_sub)
+    M = A.copy()
+    shift = rng.integers(1, 4, (n, la))
+    M[sub] = DNA_CODES[(A[sub].astype(np.int64) - 1 + shift[sub]) % 4]
+    ins = indel & (rng.random((n, la)) < 0.5)
+    dele = indel & ~ins
+    # per position: deleted -> 0 residues, inserted -> random residue then the
+    # original, else the (substituted) residue
+    count = np.where(dele, 0, np.where(ins, 2, 1))
+    mlen = count.sum(axis=1)
+    lb = np.maximum(rng.integers(lb_lo, lb_hi + 1, n), mlen)
+    left = (rng.random(n) * (lb - mlen + 1)).astype(np.int64)
+    right = lb - mlen - left
+    # flat B: left flank, mutated copy, right flank
+    cnt = count.ravel()
+    flat_m = np.repeat(M.ravel(), cnt)
+    # an inserted position emits a random residue, then its own residue
+    first_out = np.cumsum(cnt) - cnt
+    ins_at = first_out[cnt == 2]
+    flat_m[ins_at] = DNA_CODES[rng.integers(0, 4, len(ins_at))]
+    m_start = np.zeros(n, np.int64)
+    m_start[1:] = np.cumsum(mlen)[:-1]
+    B = np.empty(int(lb.sum()), np.uint8)
+    b_start = np.zeros(n, np.int64)
+    b_start[1:] = np.cumsum(lb)[:-1]
+    B[:] = DNA_CODES[rng.integers(0, 4, len(B))]  # flanks
+    dst = np.repeat(b_start + left - m_start, mlen) + np.arange(int(mlen.sum()), dtype=np.int64)
+    B[dst] = flat_m
+    return A, B, lb
+
+
 def config2(seed: int = 2, n_short: int = 1_000_000, n_long: int = 5_000) -> dict:
-    """Skewed DNA: short 150 x U[150,400] embedded pairs plus power-law long
-    pairs (alpha 1.5 over 1-16 kb) at 85% identity."""
+    """Skewed DNA: short 150 x U[150,400] embedded pairs (B = flank + mutate(A;
+    10% substitutions, 2% indels) + flank, as in config 0) plus power-law long
+    pairs (pdf ~ x^-1.5 over 1-16 kb) with B = A at 85% identity (13%
+    substitutions, 2% indels); pairs shuffled."""
     rng = np.random.default_rng(seed)
-    pairs = [dna_embedded_pair(rng, 150, int(rng.integers(150, 401))) for _ in range(n_short)]
+    A, Bflat, lb = _embedded_pairs_vec(rng, n_short, 150, 150, 400, 0.10, 0.02)
     ll = _powerlaw_len(rng, n_long, 1.5, 1000, 16000)
+    longs = []
     for k in range(n_long):
         a = _dna_draw(rng, int(ll[k]))
         b = _mutate(rng, a, 0.13, 0.02, _dna_draw, _dna_subst)
-        pairs.append((a, b))
-    order = rng.permutation(len(pairs))
-    pairs = [pairs[i] for i in order]
-    return make_batch(pairs, "dna_skewed_1M_plus_5k_long", DNA_SW, seed=seed)
+        longs.append((a, b))
+    n = n_short + n_long
+    order = rng.permutation(n)  # position of pair k in the batch
+    a_lens = np.concatenate([np.full(n_short, 150, np.int64), [len(a) for a, _ in longs]])
+    b_lens = np.concatenate([lb, [len(b) for _, b in longs]])
+    a_flat = np.concatenate([A.ravel()] + [a for a, _ in longs])
+    b_flat = np.concatenate([Bflat] + [b for _, b in longs])
+    # sequences in batch order: A_k then B_k for pair k = order^-1
+    inv = np.empty(n, np.int64)
+    inv[order] = np.arange(n)
+    a_src = np.zeros(n, np.int64); a_src[1:] = np.cumsum(a_lens)[:-1]
+    b_src = np.zeros(n, np.int64); b_src[1:] = np.cumsum(b_lens)[:-1]
+    pa = inv  # batch slot -> source pair
+    la_b, lb_b = a_lens[pa], b_lens[pa]
+    lens = np.empty(2 * n, np.int64)
+    lens[0::2], lens[1::2] = la_b, lb_b
+    src = np.empty(2 * n, np.int64)
+    src[0::2], src[1::2] = a_src[pa], b_src[pa] + len(a_flat)
+    both = np.concatenate([a_flat, b_flat])
+    gather = np.repeat(src - np.concatenate([[0], np.cumsum(lens)[:-1]]), lens) + \
+        np.arange(int(lens.sum()), dtype=np.int64)
+    codes, offs = arena_from_flat(both[gather], lens)
+    a_len = la_b.astype(np.int32); b_len = lb_b.astype(np.int32)
+    return dict(name="dna_skewed_1M_plus_5k_long", sw=DNA_SW, codes=codes, a_off=offs[0::2],
+                a_len=a_len, b_off=offs[1::2], b_len=b_len,
+                cells=int(np.dot(a_len.astype(np.int64), b_len.astype(np.int64))), seed=seed,
+                n_long=n_long)
 
 
 def random_pairs(seed, n, la_range, lb_range, protein=False, related=0.5) -> dict:

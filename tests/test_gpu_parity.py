@@ -146,7 +146,7 @@ def test_ssw_fixtures(gpu, name):
 
 
 @pytest.mark.parametrize("protein", [False, True])
-@pytest.mark.parametrize("mode", [I.BATCH_BUCKETED, I.BATCH_PLAIN])
+@pytest.mark.parametrize("mode", [I.BATCH_BUCKETED, I.BATCH_PLAIN, I.BATCH_PLAIN_TILE])
 @pytest.mark.parametrize("forward_only", [True, False])
 def test_random_batches(gpu, protein, mode, forward_only):
     note(f"random batch protein={protein} mode={mode} forward_only={forward_only}")
@@ -171,10 +171,20 @@ def test_config0_bucketed_and_plain(gpu):
     note("dna plain config (1024 pairs) bucketed vs plain vs oracle")
     batch = W.config0()
     exp = oracle_batch(batch, False)
-    for mode in (I.BATCH_BUCKETED, I.BATCH_PLAIN):
+    for mode in (I.BATCH_BUCKETED, I.BATCH_PLAIN, I.BATCH_PLAIN_TILE):
         eng = engine_for(batch["sw"], False)
         eng.set_batching(mode)
         assert_same(device_run(eng, batch), exp, f"config0 mode={mode}")
+
+
+def test_skewed_slice_all_modes(gpu):
+    note("skewed DNA slice (3000 short + 12 long pairs) in every batching mode")
+    batch = W.config2(seed=5, n_short=3000, n_long=12)
+    exp = oracle_batch(batch, True)
+    for mode in (I.BATCH_BUCKETED, I.BATCH_PLAIN_TILE, I.BATCH_PLAIN):
+        eng = engine_for(batch["sw"], True)
+        eng.set_batching(mode)
+        assert_same(device_run(eng, batch), exp, f"skewed mode={mode}")
 
 
 def test_protein_slice_forward(gpu):
