@@ -49,6 +49,12 @@ def parse():
     ap.add_argument("--with-starts", action="store_true", help="time the reverse pass too")
     ap.add_argument("--rounds", type=int, default=0,
                     help="groups per resident wave for the bucketing (0 = engine default)")
+    ap.add_argument("--serve-rates", default="200",
+                    help="comma-separated request rates (req/s) of the serving measurement")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="seconds of Poisson arrivals per rate (0 = skip serving)")
+    ap.add_argument("--serve-unbatched", action="store_true",
+                    help="also run the one-launch-per-request baseline at every rate")
     ap.add_argument("--config", choices=["protein", "skewed"], default="protein",
                     help="protein: 100k BLOSUM62 pairs (the headline); skewed: 1M short + 5k "
                          "long DNA pairs, bucketed vs the plain padded batch")
@@ -215,6 +221,20 @@ def main():
         cpu = cpu_baseline(batch, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
 
+    serving = None
+    if args.serve_seconds > 0:
+        from ipuma_amd import loadgen
+        del eng  # the measured engine's buffers are not needed any more
+        torch.cuda.empty_cache()
+        log("serving: generating request pools ...")
+        pools = loadgen.make_pools()
+        serving = []
+        for rate in [float(r) for r in args.serve_rates.split(",") if r]:
+            for merge in ([True, False] if args.serve_unbatched else [True]):
+                res = loadgen.run(pools, rate, args.serve_seconds, merge=merge)
+                log(f"serving {res}")
+                serving.append(res)
+
     tr = measured_traffic() if (args.config == "protein" and args.pairs == 100_000) else None
     out = {
         "metric": "GCUPS on 1 MI355X (bucketed vs plain, bit-exact); p50/p99 latency at fixed req/s",
@@ -244,6 +264,7 @@ def main():
                      "kernel_ms": round(fast_avg, 3),
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -187,6 +187,61 @@ int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out);
 /* Synchronize the engine stream. */
 int ipuma_synchronize(ipuma_engine *e);
 
+/*
+ * Request server: replaces the reference's batch loop over asynchronous jobs
+ * (IPUMultiDriver, ipusw/ipumultidriver.hpp:10-145; run_comparison,
+ * ipusw/run_comparison.hpp:67-172; async_submit_prepared_remote_compare /
+ * blocking_join_prepared_remote_compare, src/ipu_batch_affine.h:105-108).
+ * Requests of any size from any host thread are encoded and copied once into
+ * an HBM arena (admitted against hbm_budget_bytes), merged with other pending
+ * requests into one device batch, and retired in order; wait() returns a
+ * request's results (packed like compare_local's) and its latency from
+ * submit to results on the host.
+ */
+typedef struct ipuma_server ipuma_server;
+typedef struct ipuma_server_config {
+  int64_t hbm_budget_bytes; /* arena for resident request sequences (default 64 GiB) */
+  int32_t max_batch_pairs;  /* most comparisons per device batch, and per request (65536) */
+  int32_t max_wait_us;      /* a pending request waits at most this long while the GPU is
+                               busy before its batch is launched (2000) */
+  int32_t max_inflight;     /* device batches enqueued at once, 1..8 (2) */
+  int32_t block_on_full;    /* 1: submit waits for arena space; 0: it returns IPUMA_EBUSY */
+  int32_t merge_requests;   /* 1: pending requests share device batches (default); 0: one
+                               device batch per request (the unbatched baseline) */
+} ipuma_server_config;
+typedef struct ipuma_server_stats {
+  int64_t requests;             /* admitted */
+  int64_t batches;              /* retired device batches */
+  int64_t pairs;                /* comparisons retired */
+  int64_t cells;                /* sum |A|*|B| retired */
+  int64_t max_batch_pairs_seen; /* largest merged batch */
+  double busy_ms;               /* sum of batch launch-to-retire times */
+  int64_t arena_used_bytes;     /* HBM arena held by requests not yet retired */
+  int64_t pending_pairs;        /* comparisons waiting for a batch */
+  double span_ms;               /* first batch launch to last retire */
+} ipuma_server_stats;
+void ipuma_default_server_config(ipuma_server_config *c);
+/* One engine (sw, algo: scoring and forward_only) plus its dispatcher and
+ * completer threads on the current HIP device. */
+int ipuma_server_create(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
+                        const ipuma_server_config *cfg, ipuma_server **out);
+/* Retires everything submitted, joins the threads, frees the arena.  No call
+ * on the server may be in progress. */
+void ipuma_server_destroy(ipuma_server *s);
+/* Comparison k aligns A_k = a_chars[a_offsets[k] .. a_offsets[k+1]) against
+ * B_k likewise; the caller's buffers may be reused when this returns.  Fills
+ * *ticket (also for an upload that failed, whose wait reports the error). */
+int ipuma_server_submit(ipuma_server *s, int32_t n, const char *a_chars,
+                        const int64_t *a_offsets, const char *b_chars,
+                        const int64_t *b_offsets, int64_t *ticket);
+/* Blocks until the request is retired; copies its n results (any output may
+ * be NULL) and its latency; the ticket is then released. */
+int ipuma_server_wait(ipuma_server *s, int64_t ticket, int32_t *scores, int32_t *a_range,
+                      int32_t *b_range, double *latency_ms);
+int ipuma_server_get_stats(ipuma_server *s, ipuma_server_stats *out);
+/* The server's engine (for its batching / precision knobs; not for batches). */
+int ipuma_server_engine(ipuma_server *s, ipuma_engine **out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@
 
 #include "../../include/ipuma_hip.h"
 #include "sw_device.h"
+#include "sw_engine_internal.h"
 #include "sw_tables.h"
 
 using namespace ipuma;
@@ -152,6 +153,7 @@ int init_engine(ipuma_engine *e) {
   e->packed_ok = o >= 0 && x >= 0 && o <= 32767 && x <= 32767;
   e->forward_only = e->algo.forward_only ? 1 : 0;
   e->fast_blocks = e->num_cu * fast_blocks_per_cu(e->kp);
+  e->generic_blocks = std::max(8, 2 * e->num_cu);
   e->target_slots = (int64_t)e->fast_blocks * (kFastThreads / 64) * 128 * e->rounds;
   return IPUMA_OK;
 }
@@ -349,6 +351,21 @@ int check_sizes(ipuma_engine *e, int32_t n, const int64_t *a_lens_or_null,
 }
 
 }  // namespace
+
+namespace ipuma {
+const uint8_t *engine_code_table(const ipuma_engine *e) { return e->h_codes; }
+hipStream_t engine_stream(const ipuma_engine *e) { return e->stream; }
+int engine_check_error(ipuma_engine *e) { return check_device_error(e); }
+int set_last_error(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+}  // namespace ipuma
 
 extern "C" {
 

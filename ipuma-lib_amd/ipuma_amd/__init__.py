@@ -23,6 +23,8 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libipuma_amd.so")
 HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "ipuma_hip.h")
 
 IPUMA_OK = 0
+IPUMA_EINVAL, IPUMA_ENOMEM, IPUMA_EDEVICE, IPUMA_ECONFIG = -1, -2, -3, -4
+IPUMA_ETOOLONG, IPUMA_EBUSY, IPUMA_ECLOSED = -5, -6, -7
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ECONFIG", -5: "ETOOLONG",
           -6: "EBUSY", -7: "ECLOSED"}
 
@@ -58,6 +60,23 @@ class BatchStats(ctypes.Structure):
                 ("rerun_pairs", ctypes.c_int64), ("lanes_per_pair_max", ctypes.c_int32),
                 ("rows_per_lane", ctypes.c_int32), ("fast_kernel_ms", ctypes.c_double),
                 ("batch_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_batch_pairs", ctypes.c_int32),
+                ("max_wait_us", ctypes.c_int32), ("max_inflight", ctypes.c_int32),
+                ("block_on_full", ctypes.c_int32), ("merge_requests", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_int64), ("batches", ctypes.c_int64),
+                ("pairs", ctypes.c_int64), ("cells", ctypes.c_int64),
+                ("max_batch_pairs_seen", ctypes.c_int64), ("busy_ms", ctypes.c_double),
+                ("arena_used_bytes", ctypes.c_int64), ("pending_pairs", ctypes.c_int64),
+                ("span_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -133,6 +152,15 @@ def lib():
     L.ipuma_encode_device.argtypes = [_P, _P, _i64, _P, _P]
     L.ipuma_last_batch_stats.argtypes = [_P, ctypes.POINTER(BatchStats)]
     L.ipuma_synchronize.argtypes = [_P]
+    L.ipuma_default_server_config.argtypes = [_P]
+    L.ipuma_default_server_config.restype = None
+    L.ipuma_server_create.argtypes = [_P, _P, _P, ctypes.POINTER(_P)]
+    L.ipuma_server_destroy.argtypes = [_P]
+    L.ipuma_server_destroy.restype = None
+    L.ipuma_server_submit.argtypes = [_P, _i32, _P, _P, _P, _P, ctypes.POINTER(_i64)]
+    L.ipuma_server_wait.argtypes = [_P, _i64, _P, _P, _P, ctypes.POINTER(ctypes.c_double)]
+    L.ipuma_server_get_stats.argtypes = [_P, ctypes.POINTER(ServerStats)]
+    L.ipuma_server_engine.argtypes = [_P, ctypes.POINTER(_P)]
     return L
 
 
@@ -302,3 +330,70 @@ def unpack_range(x):
     begin = (x & 0xFFFF).astype(np.uint16).astype(np.int16)
     end = (x >> 16).astype(np.uint16).astype(np.int16)
     return begin, end
+
+
+class Server:
+    """Request server (replaces the reference's IPUMultiDriver batch loop,
+    ipusw/ipumultidriver.hpp:10-145, and its async submit / join,
+    src/ipu_batch_affine.h:105-108): submit() from any thread returns a
+    ticket, wait(ticket) the results and the latency in ms."""
+
+    def __init__(self, config: SWConfig = None, algoconfig: IPUAlgoConfig = None,
+                 hbm_budget_bytes: int = 1 << 30, max_batch_pairs: int = 1 << 16,
+                 max_wait_us: int = 2000, max_inflight: int = 2, block_on_full: bool = True,
+                 merge_requests: bool = True):
+        config = config or SWConfig()
+        algoconfig = algoconfig or IPUAlgoConfig()
+        sc = _ServerConfig(int(hbm_budget_bytes), int(max_batch_pairs), int(max_wait_us),
+                           int(max_inflight), int(block_on_full), int(merge_requests))
+        h = _P()
+        sw, al = config._c(), algoconfig._c()
+        _check(lib().ipuma_server_create(ctypes.byref(sw), ctypes.byref(al), ctypes.byref(sc),
+                                         ctypes.byref(h)))
+        self._h = h
+        self.max_batch_pairs = int(max_batch_pairs)
+
+    def submit(self, A, B) -> int:
+        if len(A) != len(B):
+            raise ValueError("A and B must have the same number of sequences")
+        abuf, aoff = _pack(A)
+        bbuf, boff = _pack(B)
+        t = _i64()
+        _check(lib().ipuma_server_submit(self._h, len(A), abuf, aoff.ctypes.data, bbuf,
+                                         boff.ctypes.data, ctypes.byref(t)))
+        return int(t.value)
+
+    def submit_raw(self, n: int, a_chars: int, a_offsets: int, b_chars: int,
+                   b_offsets: int) -> int:
+        """Zero-copy submit: host pointers to characters and to n + 1 int64
+        offsets per side (the caller keeps them alive until this returns)."""
+        t = _i64()
+        _check(lib().ipuma_server_submit(self._h, int(n), a_chars, a_offsets, b_chars,
+                                         b_offsets, ctypes.byref(t)))
+        return int(t.value)
+
+    def wait(self, ticket: int, n: int):
+        """(scores, a_range, b_range, latency_ms) of a request of n comparisons."""
+        s = np.zeros(n, np.int32)
+        ar = np.zeros(n, np.int32)
+        br = np.zeros(n, np.int32)
+        lat = ctypes.c_double()
+        _check(lib().ipuma_server_wait(self._h, int(ticket), s.ctypes.data, ar.ctypes.data,
+                                       br.ctypes.data, ctypes.byref(lat)))
+        return s, ar, br, lat.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().ipuma_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ipuma_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

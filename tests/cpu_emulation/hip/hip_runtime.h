@@ -210,6 +210,17 @@ inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, int, hipStrea
   return hipSuccess;
 }
 inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (void *)1; return hipSuccess; }
+enum { hipEventDisableTiming = 2, hipHostMallocDefault = 0 };
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = (void *)1; return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }  // work runs at launch
+inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) {
+  *p = std::malloc(n);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+template <class T> inline hipError_t hipHostMalloc(T **p, size_t n, unsigned f) {
+  return hipHostMalloc((void **)p, n, f);
+}
+inline hipError_t hipHostFree(void *p) { std::free(p); return hipSuccess; }
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }

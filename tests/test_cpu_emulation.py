@@ -52,10 +52,13 @@ def test_kernel_input_validation_asan(build_dir):
     assert r.returncode == 0, r.stdout + r.stderr
 
 
-def test_ssw_parity_pairs_emulated(build_dir):
+@pytest.fixture(scope="module")
+def emu_lib(build_dir):
+    """The engine, server, kernels and tables built against the emulation, loaded
+    as the binding's library for the duration of the module."""
     objs = []
-    for src, extra in (("sw_engine.cpp", ["-DEMU_DEFINE_GLOBALS"]), ("sw_kernels.hip", []),
-                       ("sw_tables.cpp", [])):
+    for src, extra in (("sw_engine.cpp", ["-DEMU_DEFINE_GLOBALS"]), ("sw_server.cpp", []),
+                       ("sw_kernels.hip", []), ("sw_tables.cpp", [])):
         o = os.path.join(build_dir, src + ".o")
         _compile(os.path.join(CSRC, src), o, extra)
         objs.append(o)
@@ -67,19 +70,75 @@ def test_ssw_parity_pairs_emulated(build_dir):
     I.lib.cache_clear()
     saved = I.LIB_PATH
     I.LIB_PATH = lib_path
-    try:
-        d = json.load(open(os.path.join(HERE, "golden", "ssw_fixtures.json")))
-        case = d["cases"][0]  # 18 DNA pairs of ~100 bp, 8 lanes per pair
-        c = case["config"]
-        eng = I.SWAlgorithm(I.SWConfig(gapInit=c["gap_init"], gapExtend=c["gap_extend"]),
-                            I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
-        s, ar, br = eng.compare([p[0] for p in case["pairs"]], [p[1] for p in case["pairs"]])
-        qb, qe = I.unpack_range(ar)
-        rb, re_ = I.unpack_range(br)
-        for k, (score, eqb, eqe, erb, ere) in enumerate(case["expected"]):
-            assert (int(s[k]), int(qb[k]), int(qe[k]), int(rb[k]), int(re_[k])) == \
-                (score, eqb, eqe, erb, ere), k
-        eng.close()
-    finally:
-        I.LIB_PATH = saved
-        I.lib.cache_clear()
+    yield I
+    I.LIB_PATH = saved
+    I.lib.cache_clear()
+
+
+def test_ssw_parity_pairs_emulated(emu_lib):
+    I = emu_lib
+    d = json.load(open(os.path.join(HERE, "golden", "ssw_fixtures.json")))
+    case = d["cases"][0]  # 18 DNA pairs of ~100 bp, 8 lanes per pair
+    c = case["config"]
+    eng = I.SWAlgorithm(I.SWConfig(gapInit=c["gap_init"], gapExtend=c["gap_extend"]),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    s, ar, br = eng.compare([p[0] for p in case["pairs"]], [p[1] for p in case["pairs"]])
+    qb, qe = I.unpack_range(ar)
+    rb, re_ = I.unpack_range(br)
+    for k, (score, eqb, eqe, erb, ere) in enumerate(case["expected"]):
+        assert (int(s[k]), int(qb[k]), int(qe[k]), int(rb[k]), int(re_[k])) == \
+            (score, eqb, eqe, erb, ere), k
+    eng.close()
+
+
+def test_request_server_emulated(emu_lib):
+    """Three submitting threads, a 2 KiB arena (so admission blocks and the ring
+    wraps), results checked against the oracle; then a request larger than the
+    arena is refused with IPUMA_EBUSY when blocking is off."""
+    import threading
+    import numpy as np
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(4)
+    reqs = []
+    for r in range(6):
+        k = int(rng.integers(1, 7))
+        A = ["".join(rng.choice(list("ACGT"), int(rng.integers(1, 60)))) for _ in range(k)]
+        B = ["".join(rng.choice(list("ACGT"), int(rng.integers(1, 60)))) + a[: len(a) // 2]
+             for a in A]
+        reqs.append((A, B))
+    srv = I.Server(I.SWConfig(gapInit=-2, gapExtend=-1), I.IPUAlgoConfig(forwardOnly=True),
+                   hbm_budget_bytes=2048, max_batch_pairs=16, max_wait_us=500)
+    out = [None] * len(reqs)
+    errors = []
+
+    def worker(idx):
+        try:
+            for i in idx:
+                t = srv.submit(*reqs[i])
+                out[i] = srv.wait(t, len(reqs[i][0]))
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(list(range(j, len(reqs), 3)),))
+               for j in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=600)
+    assert not errors, errors
+    for (A, B), got in zip(reqs, out):
+        exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1, forward_only=True)
+        for g, e in zip(got[:3], exp):
+            assert np.array_equal(g, e)
+        assert got[3] >= 0.0
+    st = srv.stats()
+    assert st["requests"] == len(reqs) and st["pairs"] == sum(len(a) for a, _ in reqs)
+    assert st["arena_used_bytes"] == 0 and st["pending_pairs"] == 0
+    srv.close()
+
+    srv = I.Server(I.SWConfig(), I.IPUAlgoConfig(), hbm_budget_bytes=2048, block_on_full=False)
+    with pytest.raises(I.IpumaError) as ei:
+        srv.submit(["A" * 3000], ["C" * 3000])
+    assert ei.value.code == I.IPUMA_EBUSY
+    srv.close()

@@ -270,3 +270,40 @@ def test_cpp_facade(gpu):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     print(r.stdout[-3000:])
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+
+
+def test_request_server(gpu):
+    note("request server: 4 threads x 6 DNA requests of 1-300 pairs vs the oracle")
+    import threading
+    rng = np.random.default_rng(21)
+    reqs = []
+    for r in range(24):
+        b = W.config0(seed=100 + r, n=int(rng.integers(1, 301)))
+        A = [W.decode(b["codes"][o:o + l], False) for o, l in zip(b["a_off"], b["a_len"])]
+        B = [W.decode(b["codes"][o:o + l], False) for o, l in zip(b["b_off"], b["b_len"])]
+        reqs.append((A, B))
+    srv = I.Server(I.SWConfig(**W.DNA_SW), I.IPUAlgoConfig(forwardOnly=False),
+                   hbm_budget_bytes=1 << 20, max_batch_pairs=4096, max_wait_us=1000)
+    out = [None] * len(reqs)
+    errors = []
+
+    def worker(idx):
+        try:
+            for i in idx:
+                out[i] = srv.wait(srv.submit(*reqs[i]), len(reqs[i][0]))
+        except Exception as e:
+            errors.append(e)
+
+    ths = [threading.Thread(target=worker, args=(list(range(j, len(reqs), 4)),)) for j in range(4)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join(timeout=120)
+    assert not errors, errors
+    for (A, B), got in zip(reqs, out):
+        exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1)
+        for g, e in zip(got[:3], exp):
+            assert np.array_equal(g, e)
+    st = srv.stats()
+    assert st["requests"] == len(reqs) and st["arena_used_bytes"] == 0
+    srv.close()
