@@ -140,6 +140,5 @@ def run(pools, rate: float, duration: float, max_pairs: int = 4096, merge: bool 
         "gcups": round(float(cells[ok].sum() / max(span, 1e-9) / 1e9), 2),
         "offered_gcups": round(float(cells.sum() / duration / 1e9), 2),
         "batches": int(batches), "mean_batch_pairs": round(pairs / max(batches, 1), 1),
-        "device_busy_frac": round(sum(v["busy_ms"] for v in st.values()) / 1e3 / max(span, 1e-9), 3),
         "merged": bool(merge),
     }
