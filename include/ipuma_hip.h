@@ -136,6 +136,22 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
                          int32_t *scores, int32_t *a_range, int32_t *b_range,
                          void *stream);
 
+/*
+ * One-vs-many search (the database-search use of compare_local, with every
+ * query against every database sequence): comparison q * ndb + d aligns
+ * database sequence d (A) against query q (B); all sequences are in one
+ * device arena of codes (offsets 4-aligned, as for ipuma_compare_device).
+ * Writes every score (scores: nq * ndb, or NULL) and per query the top k
+ * (k <= 1024) database indices and scores, ordered by score descending, then
+ * index ascending (top_idx / top_score: nq * k; -1 / 0 past ndb).  Forward
+ * pass only.  No reference counterpart: the reference expresses search as
+ * compare_local / compare_mn_local over explicit pairs (src/ipu_batch_affine.h:99-100).
+ */
+int ipuma_search_device(ipuma_engine *e, const uint8_t *codes, int64_t codes_len, int32_t nq,
+                        const int64_t *q_off, const int32_t *q_len, int32_t ndb,
+                        const int64_t *db_off, const int32_t *db_len, int32_t k,
+                        int32_t *scores, int32_t *top_idx, int32_t *top_score, void *stream);
+
 /* Batching policy of the device path (replaces IPUAlgoConfig.fillAlgo's
  * partition::fillFirst/roundRobin/greedy, src/partition.cpp:197-462). */
 enum {

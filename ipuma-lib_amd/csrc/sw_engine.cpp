@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -110,6 +111,7 @@ struct ipuma_engine {
   DevBuf scratch, work;
   ChainBufs fwd, rev;
   DevBuf rcodes, rlen, rres;
+  DevBuf s_pairs, s_res;  // one-vs-many search: pair descriptors and results
   // host staging of compare_local / compare_mn_local
   DevBuf in_chars, in_codes, in_meta, out_res;
   std::vector<char> h_chars;
@@ -427,6 +429,8 @@ void ipuma_engine_destroy(ipuma_engine *e) {
   e->rcodes.release();
   e->rlen.release();
   e->rres.release();
+  e->s_pairs.release();
+  e->s_res.release();
   e->in_chars.release();
   e->in_codes.release();
   e->in_meta.release();
@@ -523,6 +527,40 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : e->stream;
   return compare_device_impl(e, n, codes, codes_len, a_off, a_len, b_off, b_len, scores, a_range,
                              b_range, st);
+}
+
+int ipuma_search_device(ipuma_engine *e, const uint8_t *codes, int64_t codes_len, int32_t nq,
+                        const int64_t *q_off, const int32_t *q_len, int32_t ndb,
+                        const int64_t *db_off, const int32_t *db_len, int32_t k,
+                        int32_t *scores, int32_t *top_idx, int32_t *top_score, void *stream) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  if (nq < 0 || ndb < 0 || k < 0 || k > 1024)
+    return fail(IPUMA_EINVAL, "nq, ndb >= 0 and 0 <= k <= 1024 required");
+  const int64_t n = (int64_t)nq * ndb;
+  if (n > INT32_MAX - kSortedPad) return fail(IPUMA_EINVAL, "nq * ndb too large");
+  if (n > 0 && (!codes || !q_off || !q_len || !db_off || !db_len))
+    return fail(IPUMA_EINVAL, "NULL buffer");
+  if (k > 0 && (!top_idx || !top_score)) return fail(IPUMA_EINVAL, "NULL top-k buffer");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : e->stream;
+  int rc;
+  if ((rc = e->s_pairs.ensure((size_t)std::max<int64_t>(n, 1) * 24)) ||
+      (rc = e->s_res.ensure((size_t)std::max<int64_t>(n, 1) * 12)))
+    return rc;
+  int64_t *a_off = e->s_pairs.as<int64_t>(), *b_off = a_off + n;
+  int32_t *a_len = reinterpret_cast<int32_t *>(b_off + n), *b_len = a_len + n;
+  int32_t *res = e->s_res.as<int32_t>();
+  int32_t *sc = scores ? scores : res + 2 * n;
+  HIP_TRY(launch_search_pairs(nq, ndb, q_off, q_len, db_off, db_len, a_off, a_len, b_off, b_len,
+                              st));
+  // scores only: the forward pass (its end positions are discarded)
+  const int saved = e->forward_only;
+  e->forward_only = 1;
+  rc = compare_device_impl(e, (int32_t)n, codes, codes_len, a_off, a_len, b_off, b_len, sc, res,
+                           res + n, st);
+  e->forward_only = saved;
+  if (rc) return rc;
+  HIP_TRY(launch_topk(nq, ndb, k, sc, top_idx, top_score, st));
+  return IPUMA_OK;
 }
 
 int ipuma_set_batching(ipuma_engine *e, int32_t mode) {

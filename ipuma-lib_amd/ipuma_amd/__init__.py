@@ -152,6 +152,8 @@ def lib():
     L.ipuma_encode_device.argtypes = [_P, _P, _i64, _P, _P]
     L.ipuma_last_batch_stats.argtypes = [_P, ctypes.POINTER(BatchStats)]
     L.ipuma_synchronize.argtypes = [_P]
+    L.ipuma_search_device.argtypes = [_P, _P, _i64, _i32, _P, _P, _i32, _P, _P, _i32, _P, _P, _P,
+                                      _P]
     L.ipuma_default_server_config.argtypes = [_P]
     L.ipuma_default_server_config.restype = None
     L.ipuma_server_create.argtypes = [_P, _P, _P, ctypes.POINTER(_P)]
@@ -309,6 +311,20 @@ class SWAlgorithm:
         _check(lib().ipuma_compare_device(
             self._h, n, ptr(codes), int(codes_len), ptr(a_off), ptr(a_len), ptr(b_off),
             ptr(b_len), ptr(scores), ptr(a_range), ptr(b_range), stream))
+
+    def search_device(self, codes, codes_len, q_off, q_len, db_off, db_len, k, scores,
+                      top_idx, top_score, stream=None):
+        """Every query against every database sequence (device pointers or tensors);
+        scores may be None (only the top k per query is kept)."""
+        def ptr(x):
+            if x is None or isinstance(x, int):
+                return x
+            return x.ctypes.data if isinstance(x, np.ndarray) else x.data_ptr()
+        nq = int(q_len.numel()) if hasattr(q_len, "numel") else int(len(q_len))
+        ndb = int(db_len.numel()) if hasattr(db_len, "numel") else int(len(db_len))
+        _check(lib().ipuma_search_device(
+            self._h, ptr(codes), int(codes_len), nq, ptr(q_off), ptr(q_len), ndb, ptr(db_off),
+            ptr(db_len), int(k), ptr(scores), ptr(top_idx), ptr(top_score), stream))
 
     def encode_device(self, chars, length, codes, stream=None):
         def ptr(x):

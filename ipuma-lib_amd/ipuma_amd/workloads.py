@@ -300,6 +300,46 @@ def config2(seed: int = 2, n_short: int = 1_000_000, n_long: int = 5_000) -> dic
                 n_long=n_long)
 
 
+CONFIG4_QUERY_LENGTHS = (128, 256, 512, 768, 1024, 1536, 2048, 4096)
+
+
+def config4(seed: int = 4, ndb: int = 200_000, qlens=CONFIG4_QUERY_LENGTHS,
+            homolog_frac: float = 0.001) -> dict:
+    """One-vs-many protein search: queries of the given lengths against a
+    database with lognormal(350, 0.5) lengths clipped to [30, 5000]; a fraction
+    of the database are planted homologs of a random query at 30-60% identity
+    (BLOSUM62-weighted substitutions, 5% indels).  All residues follow the
+    BLOSUM62 background.  Returns one arena with the queries first."""
+    rng = np.random.default_rng(seed)
+    queries = [_aa_draw(rng, int(L)) for L in qlens]
+    dl = _lognormal_len(rng, ndb, 350, 0.5, 30, 5000)
+    nh = int(round(ndb * homolog_frac))
+    hidx = rng.choice(ndb, nh, replace=False)
+    planted = {}
+    for i in hidx:
+        q = queries[int(rng.integers(0, len(queries)))]
+        ident = rng.uniform(0.3, 0.6)
+        h = _mutate(rng, q, 1.0 - ident, 0.05, _aa_draw, _aa_subst)[:5000]
+        if len(h) < 30:
+            h = np.concatenate([h, _aa_draw(rng, 30 - len(h))])
+        planted[int(i)] = h
+        dl[i] = len(h)
+    flat = _aa_draw(rng, int(dl.sum()))
+    starts = np.concatenate([[0], np.cumsum(dl)[:-1]])
+    for i, h in planted.items():
+        flat[starts[i]:starts[i] + len(h)] = h
+    qflat = np.concatenate(queries)
+    lens = np.concatenate([np.array(qlens, np.int64), dl])
+    codes, offs = arena_from_flat(np.concatenate([qflat, flat]), lens)
+    nq = len(qlens)
+    q_len = np.array(qlens, np.int32)
+    db_len = dl.astype(np.int32)
+    return dict(name="protein_search_8q_x_200k", sw=PROTEIN_SW, codes=codes,
+                q_off=offs[:nq], q_len=q_len, db_off=offs[nq:], db_len=db_len,
+                cells=int(q_len.astype(np.int64).sum() * db_len.astype(np.int64).sum()),
+                homologs=np.sort(hidx), seed=seed)
+
+
 def random_pairs(seed, n, la_range, lb_range, protein=False, related=0.5) -> dict:
     """Small random mix used by the parity tests."""
     rng = np.random.default_rng(seed)

@@ -13,6 +13,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -58,7 +59,7 @@ def emu_lib(build_dir):
     as the binding's library for the duration of the module."""
     objs = []
     for src, extra in (("sw_engine.cpp", ["-DEMU_DEFINE_GLOBALS"]), ("sw_server.cpp", []),
-                       ("sw_kernels.hip", []), ("sw_tables.cpp", [])):
+                       ("sw_kernels.hip", []), ("sw_search.hip", []), ("sw_tables.cpp", [])):
         o = os.path.join(build_dir, src + ".o")
         _compile(os.path.join(CSRC, src), o, extra)
         objs.append(o)
@@ -142,3 +143,48 @@ def test_request_server_emulated(emu_lib):
         srv.submit(["A" * 3000], ["C" * 3000])
     assert ei.value.code == I.IPUMA_EBUSY
     srv.close()
+
+
+def _topk_reference(scores, k):
+    """Top k of every row: score descending, then index ascending."""
+    out_i, out_s = [], []
+    for row in scores:
+        order = np.lexsort((np.arange(len(row)), -row.astype(np.int64)))[:k]
+        out_i.append(np.concatenate([order, -np.ones(k - len(order), np.int64)]))
+        out_s.append(np.concatenate([row[order], np.zeros(k - len(order), np.int64)]))
+    return np.array(out_i), np.array(out_s)
+
+
+def test_search_topk_emulated(emu_lib):
+    """Every query against every database sequence, with the per-query top k:
+    short DNA (many tied scores, so the index tie-break matters), a k that is
+    not a power of two, and a k larger than the database."""
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(8)
+    qs = [rng.choice([1, 2, 3, 4], int(rng.integers(5, 40))).astype(np.uint8) for _ in range(3)]
+    db = [rng.choice([1, 2, 3, 4], int(rng.integers(1, 30))).astype(np.uint8) for _ in range(120)]
+    from ipuma_amd.workloads import arena_from_flat
+    lens = np.array([len(x) for x in qs + db], np.int64)
+    codes, offs = arena_from_flat(np.concatenate(qs + db), lens)
+    q_off, db_off = offs[:3].copy(), offs[3:].copy()
+    q_len = lens[:3].astype(np.int32)
+    db_len = lens[3:].astype(np.int32)
+    eng = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    mat = O.select_matrix(O.SIM_NA, 1, -1, -1)
+    exp = np.zeros((3, 120), np.int64)
+    for qi in range(3):
+        s, _, _ = O.sw_batch_codes(codes, db_off, db_len, np.full(120, q_off[qi]),
+                                   np.full(120, q_len[qi], np.int32), mat, -2, -1, True)
+        exp[qi] = s
+    for k in (17, 150):
+        scores = np.zeros(3 * 120, np.int32)
+        ti = np.zeros(3 * k, np.int32)
+        ts = np.zeros(3 * k, np.int32)
+        eng.search_device(codes, len(codes), q_off, q_len, db_off, db_len, k, scores, ti, ts)
+        assert np.array_equal(scores.reshape(3, 120), exp)
+        ri, rs = _topk_reference(exp, k)
+        assert np.array_equal(ti.reshape(3, k), ri), k
+        assert np.array_equal(ts.reshape(3, k), rs), k
+    eng.close()

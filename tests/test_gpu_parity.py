@@ -307,3 +307,33 @@ def test_request_server(gpu):
     st = srv.stats()
     assert st["requests"] == len(reqs) and st["arena_used_bytes"] == 0
     srv.close()
+
+
+def test_search_top100(gpu):
+    note("one-vs-many search: 4 protein queries x 2500 database sequences, top 100")
+    import torch
+    b = W.config4(seed=9, ndb=2500, qlens=(50, 120, 300, 700), homolog_frac=0.01)
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(np.ascontiguousarray(b[k])).to(dev)
+         for k in ("codes", "q_off", "q_len", "db_off", "db_len")}
+    nq, ndb, k = 4, 2500, 100
+    scores = torch.empty(nq * ndb, dtype=torch.int32, device=dev)
+    ti = torch.empty(nq * k, dtype=torch.int32, device=dev)
+    ts = torch.empty(nq * k, dtype=torch.int32, device=dev)
+    eng = I.SWAlgorithm(I.SWConfig(**W.PROTEIN_SW),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    eng.search_device(t["codes"], t["codes"].numel(), t["q_off"], t["q_len"], t["db_off"],
+                      t["db_len"], k, scores, ti, ts)
+    eng.synchronize()
+    got = scores.cpu().numpy().reshape(nq, ndb)
+    mat = O.select_matrix(O.SIM_BLOSUM62)
+    for q in range(nq):
+        exp, _, _ = O.sw_batch_codes(b["codes"], b["db_off"], b["db_len"],
+                                     np.full(ndb, b["q_off"][q]), np.full(ndb, b["q_len"][q],
+                                                                          np.int32),
+                                     mat, -10, -1, True)
+        assert np.array_equal(got[q], exp), q
+        order = np.lexsort((np.arange(ndb), -exp.astype(np.int64)))[:k]
+        assert np.array_equal(ti.cpu().numpy().reshape(nq, k)[q], order), q
+        assert np.array_equal(ts.cpu().numpy().reshape(nq, k)[q], exp[order]), q
+    eng.close()
