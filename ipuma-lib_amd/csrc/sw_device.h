@@ -115,12 +115,14 @@ struct BucketArgs {
                           // each group padded to its own longest pair
   int32_t max_groups;
   int32_t *bin_of;
+  int32_t *rank;          // n: slot of the pair inside its bin
   int32_t *hist;          // kNumBins
   int32_t *costhist;      // kCostBins
   BucketCtl *ctl;
   int32_t *sorted;        // n + kSortedPad, pre-filled with -1
   GroupDesc *gtmp;
   int32_t *gbin;
+  int32_t *grank;         // max_groups: slot of the group inside its cost bin
   GroupDesc *groups;
   int32_t *scores;
   int32_t *a_range;

@@ -78,9 +78,10 @@ struct DevBuf {
 // Device state of one bucketing + kernel chain (the forward pass and the
 // reverse pass each own one, so they can be in flight together).
 struct ChainBufs {
-  DevBuf bin_of, hist, costhist, ctl, sorted, gtmp, gbin, groups, ovf;
+  DevBuf bin_of, rank, hist, costhist, ctl, sorted, gtmp, gbin, grank, groups, ovf;
   void release() {
-    for (DevBuf *b : {&bin_of, &hist, &costhist, &ctl, &sorted, &gtmp, &gbin, &groups, &ovf})
+    for (DevBuf *b : {&bin_of, &rank, &hist, &costhist, &ctl, &sorted, &gtmp, &gbin, &grank,
+                      &groups, &ovf})
       b->release();
   }
 };
@@ -164,7 +165,8 @@ int ensure_chain(ChainBufs &c, int32_t n) {
   int rc;
   const size_t nn = (size_t)std::max(n, 1);
   const size_t max_groups = nn + kRuns;  // at most one part-filled group per run
-  if ((rc = c.bin_of.ensure(nn * 4)) || (rc = c.hist.ensure((size_t)kNumBins * 4)) ||
+  if ((rc = c.bin_of.ensure(nn * 4)) || (rc = c.rank.ensure(nn * 4)) ||
+      (rc = c.grank.ensure(max_groups * 4)) || (rc = c.hist.ensure((size_t)kNumBins * 4)) ||
       (rc = c.costhist.ensure((size_t)kCostBins * 4)) || (rc = c.ctl.ensure(sizeof(BucketCtl))) ||
       (rc = c.sorted.ensure((nn + kSortedPad) * 4)) || (rc = c.gtmp.ensure(max_groups * sizeof(GroupDesc))) ||
       (rc = c.gbin.ensure(max_groups * 4)) || (rc = c.groups.ensure(max_groups * sizeof(GroupDesc))) ||
@@ -203,7 +205,8 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   BucketArgs B{};
   B.n = n; B.a_len = a_len; B.b_len = b_len; B.target_slots = e->target_slots;
   B.force_class = e->force_class; B.plain = e->batching == IPUMA_BATCH_PLAIN ? 1 : e->batching == IPUMA_BATCH_PLAIN_TILE ? 2 : 0;
-  B.max_groups = n + kRuns; B.bin_of = c.bin_of.as<int32_t>(); B.hist = c.hist.as<int32_t>();
+  B.max_groups = n + kRuns; B.bin_of = c.bin_of.as<int32_t>(); B.rank = c.rank.as<int32_t>();
+  B.hist = c.hist.as<int32_t>(); B.grank = c.grank.as<int32_t>();
   B.costhist = c.costhist.as<int32_t>(); B.ctl = ctl; B.sorted = c.sorted.as<int32_t>();
   B.gtmp = c.gtmp.as<GroupDesc>(); B.gbin = c.gbin.as<int32_t>(); B.groups = c.groups.as<GroupDesc>();
   B.scores = scores; B.a_range = a_range; B.b_range = b_range;

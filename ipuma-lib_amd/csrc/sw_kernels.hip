@@ -549,15 +549,33 @@ __global__ __launch_bounds__(256) void k_sum_cells(int32_t n, const int32_t *a_l
   }
 }
 
-__global__ __launch_bounds__(256) void k_classify(int32_t n, const int32_t *a_len,
-                                                  const int32_t *b_len, int64_t target_slots,
-                                                  int32_t force_class, int32_t *bin_of,
-                                                  int32_t *hist, BucketCtl *ctl, int32_t *scores,
-                                                  int32_t *a_range, int32_t *b_range) {
+// Per-pair bin and rank.  Device-scope atomics run at about 2 G/s over the
+// whole chip, so each block first counts its pairs per bin in an LDS hash
+// table (the pair's rank inside the block comes back from the LDS atomic),
+// then adds one count per distinct bin to the global histogram; the value that
+// atomic returns is the block's first slot in the bin.  rank[p] is then the
+// pair's slot inside its bin, and k_scatter places pairs without atomics.
+constexpr int kClsThreads = 1024;
+constexpr int kClsHash = 2048;  // at most kClsThreads keys: load factor <= 1/2
+__global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32_t *a_len,
+                                                          const int32_t *b_len,
+                                                          int64_t target_slots,
+                                                          int32_t force_class, int32_t *bin_of,
+                                                          int32_t *rank, int32_t *hist,
+                                                          BucketCtl *ctl, int32_t *scores,
+                                                          int32_t *a_range, int32_t *b_range) {
   __shared__ int s_count[kNumClasses];
-  if (threadIdx.x < kNumClasses) s_count[threadIdx.x] = 0;
+  __shared__ int32_t s_key[kClsHash];
+  __shared__ int32_t s_cnt[kClsHash];
+  const int tid = threadIdx.x;
+  if (tid < kNumClasses) s_count[tid] = 0;
+  for (int i = tid; i < kClsHash; i += blockDim.x) {
+    s_key[i] = -1;
+    s_cnt[i] = 0;
+  }
   __syncthreads();
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.x * blockDim.x + tid;
+  int bin = -1;
   if (p < n) {
     const int m = a_len[p], nb = b_len[p];
     if (m < 0 || nb < 0 || m > kMaxSeqLen || nb > kMaxSeqLen) {
@@ -586,22 +604,36 @@ __global__ __launch_bounds__(256) void k_classify(int32_t n, const int32_t *a_le
                                                       (unsigned long long)(kRows * target_slots)));
         cls = pow2_class((m + 1023) / 1024);
         while (cls < kNumClasses - 1) {
-          const int64_t Lc = kClassL[cls];
+          const int Lc = kClassL[cls];
           if ((int64_t)((nb + Lc * kRows - 1) / (Lc * kRows)) * (m + Lc - 1) <= cap) break;
           ++cls;
         }
       }
       const int L = kClassL[cls];
       const int nss = (nb + L * kRows - 1) / (L * kRows);
-      const int bin = (cls * kNssBins + nss_bin_of(nss)) * kMBins + mbin_of(m);
+      bin = (cls * kNssBins + nss_bin_of(nss)) * kMBins + mbin_of(m);
       bin_of[p] = bin;
-      atomicAdd(&hist[bin], 1);
       atomicAdd(&s_count[cls], 1);
     }
   }
+  int slot = 0, local = 0;
+  if (bin >= 0) {
+    slot = (int)(((uint32_t)bin * 2654435761u) >> 21);  // 11 bits: kClsHash
+    for (;;) {
+      const int old = atomicCAS(&s_key[slot], -1, bin);
+      if (old == -1 || old == bin) break;
+      slot = (slot + 1) & (kClsHash - 1);
+    }
+    local = atomicAdd(&s_cnt[slot], 1);
+  }
   __syncthreads();
-  if (threadIdx.x < kNumClasses && s_count[threadIdx.x])
-    atomicAdd(&ctl->class_count[threadIdx.x], s_count[threadIdx.x]);
+  for (int i = tid; i < kClsHash; i += blockDim.x) {
+    const int k = s_key[i];
+    if (k >= 0) s_cnt[i] = atomicAdd(&hist[k], s_cnt[i]);  // count -> the block's base
+  }
+  __syncthreads();
+  if (bin >= 0) rank[p] = s_cnt[slot] + local;
+  if (tid < kNumClasses && s_count[tid]) atomicAdd(&ctl->class_count[tid], s_count[tid]);
 }
 
 // Single block: exclusive scan of the bins with every run (class, super-strip
@@ -609,44 +641,81 @@ __global__ __launch_bounds__(256) void k_classify(int32_t n, const int32_t *a_le
 // run.  Thread t owns the 128 bins [128 t, 128 t + 128): half of run t / 2.
 constexpr int kScanThreads = kNumBins / 128;
 static_assert(kNumBins % 128 == 0 && kMBins == 256 && kScanThreads <= 1024, "scan layout");
+static_assert(kRuns <= kScanThreads && kRuns % 64 == 0, "one thread per run");
 __global__ __launch_bounds__(kScanThreads) void k_scan_bins(int32_t *hist, BucketCtl *ctl) {
   __shared__ int32_t part[kScanThreads];
   __shared__ int32_t rstart[kRuns];
+  __shared__ int32_t wsum[2][kRuns / 64];
   const int tid = threadIdx.x;
+  int4 *h4 = reinterpret_cast<int4 *>(hist) + tid * 32;
   int sum = 0;
-  for (int i = 0; i < 128; ++i) sum += hist[tid * 128 + i];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    const int4 v = h4[i];
+    sum += v.x + v.y + v.z + v.w;
+  }
   part[tid] = sum;
   __syncthreads();
-  if (tid == 0) {
-    int aligned = 0, gb = 0;
-    for (int r = 0; r < kRuns; ++r) {
-      const int cnt = part[2 * r] + part[2 * r + 1];
-      const int gs = 128 / kClassL[r / kNssBins];
-      rstart[r] = aligned;
-      ctl->run_start[r] = aligned;
-      ctl->run_group_base[r] = gb;
-      gb += (cnt + gs - 1) / gs;
-      aligned += (cnt + kRunAlign - 1) / kRunAlign * kRunAlign;
+  // one thread per run: its aligned size and group count, scanned over the
+  // runs (wave scans, then the wave totals)
+  int al = 0, ng = 0, inc_al = 0, inc_ng = 0;
+  if (tid < kRuns) {
+    const int cnt = part[2 * tid] + part[2 * tid + 1];
+    const int gs = 128 / kClassL[tid / kNssBins];
+    ng = (cnt + gs - 1) / gs;
+    al = (cnt + kRunAlign - 1) / kRunAlign * kRunAlign;
+    inc_al = al;
+    inc_ng = ng;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int ua = __shfl_up(inc_al, off), un = __shfl_up(inc_ng, off);
+      if ((tid & 63) >= off) {
+        inc_al += ua;
+        inc_ng += un;
+      }
     }
-    ctl->run_group_base[kRuns] = gb;
-    ctl->ngroups = gb;
+    if ((tid & 63) == 63) {
+      wsum[0][tid >> 6] = inc_al;
+      wsum[1][tid >> 6] = inc_ng;
+    }
+  }
+  __syncthreads();
+  if (tid < kRuns) {
+    int pa = 0, pn = 0;
+    for (int w = 0; w < (tid >> 6); ++w) {
+      pa += wsum[0][w];
+      pn += wsum[1][w];
+    }
+    const int start = pa + inc_al - al, gbase = pn + inc_ng - ng;
+    rstart[tid] = start;
+    ctl->run_start[tid] = start;
+    ctl->run_group_base[tid] = gbase;
+    if (tid == kRuns - 1) {
+      ctl->run_group_base[kRuns] = gbase + ng;
+      ctl->ngroups = gbase + ng;
+    }
   }
   __syncthreads();
   int pos = rstart[tid >> 1] + ((tid & 1) ? part[tid - 1] : 0);
-  for (int i = 0; i < 128; ++i) {
-    const int b = tid * 128 + i;
-    const int h = hist[b];
-    hist[b] = pos;
-    pos += h;
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    const int4 v = h4[i];
+    int4 o;
+    o.x = pos;
+    o.y = pos + v.x;
+    o.z = o.y + v.y;
+    o.w = o.z + v.z;
+    pos = o.w + v.w;
+    h4[i] = o;
   }
 }
 
-__global__ void k_scatter(int32_t n, const int32_t *bin_of, int32_t *offsets, int32_t *sorted) {
+__global__ void k_scatter(int32_t n, const int32_t *bin_of, const int32_t *rank,
+                          const int32_t *offsets, int32_t *sorted) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int b = bin_of[p];
   if (b < 0) return;
-  sorted[atomicAdd(&offsets[b], 1)] = p;
+  sorted[offsets[b] + rank[p]] = p;
 }
 
 __device__ __forceinline__ int cost_bin(int64_t cost) {
@@ -656,49 +725,78 @@ __device__ __forceinline__ int cost_bin(int64_t cost) {
   return min(kCostBinsPerClass - 1, (int)(l2 * 8.0f));
 }
 
-// One wave per group: its run (binary search over the run group bases), its
-// slots, and the padded shape over the pairs it holds.
-__global__ __launch_bounds__(256) void k_groups(int32_t max_groups, const int32_t *sorted,
-                                                const int32_t *a_len, const int32_t *b_len,
-                                                BucketCtl *ctl, GroupDesc *gtmp, int32_t *gbin,
-                                                int32_t *costhist) {
-  const int gi = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int lane = threadIdx.x & 63;
-  if (gi >= max_groups || gi >= ctl->ngroups) return;
-  int lo = 0, hi = kRuns - 1;  // last run whose first group is <= gi
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ctl->run_group_base[mid] <= gi) lo = mid; else hi = mid - 1;
-  }
-  const int c = lo / kNssBins;
-  const int L = kClassL[c];
-  const int gs = 128 / L;
-  const int base = ctl->run_start[lo] + (gi - ctl->run_group_base[lo]) * gs;
-  int mpad = 0, nbmax = 0;
-  for (int s = lane; s < gs; s += 64) {
-    const int p = sorted[base + s];
-    if (p >= 0) {
-      mpad = max(mpad, a_len[p]);
-      nbmax = max(nbmax, b_len[p]);
+// One wave per group, grid-stride over the groups: its run (binary search
+// over the run group bases), its slots, and the padded shape over the pairs it
+// holds.  The cost bins are counted per block in LDS (one global atomic per
+// bin and block, whose result is the block's base in the bin), so
+// k_order_groups places groups without atomics.
+constexpr int kGroupThreads = 1024;
+constexpr int kGroupBlocks = 128;
+__global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, const int32_t *sorted,
+                                                          const int32_t *a_len,
+                                                          const int32_t *b_len, BucketCtl *ctl,
+                                                          GroupDesc *gtmp, int32_t *gbin,
+                                                          int32_t *grank, int32_t *costhist) {
+  __shared__ int32_t s_cost[kCostBins];
+  __shared__ unsigned long long s_pad[kGroupThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int i = tid; i < kCostBins; i += blockDim.x) s_cost[i] = 0;
+  __syncthreads();
+  const int ng = min(max_groups, ctl->ngroups);
+  const int nw = (int)(blockDim.x >> 6);
+  const int stride = (int)gridDim.x * nw;
+  unsigned long long pad = 0;
+  for (int gi = (int)blockIdx.x * nw + w; gi < ng; gi += stride) {
+    int lo = 0, hi = kRuns - 1;  // last run whose first group is <= gi
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (ctl->run_group_base[mid] <= gi) lo = mid; else hi = mid - 1;
+    }
+    const int c = lo / kNssBins;
+    const int L = kClassL[c];
+    const int gs = 128 / L;
+    const int base = ctl->run_start[lo] + (gi - ctl->run_group_base[lo]) * gs;
+    int mpad = 0, nbmax = 0;
+    for (int s = lane; s < gs; s += 64) {
+      const int p = sorted[base + s];
+      if (p >= 0) {
+        mpad = max(mpad, a_len[p]);
+        nbmax = max(nbmax, b_len[p]);
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      mpad = max(mpad, __shfl_xor(mpad, off));
+      nbmax = max(nbmax, __shfl_xor(nbmax, off));
+    }
+    if (lane == 0) {
+      GroupDesc d;
+      d.cls = c;
+      d.mpad = mpad;
+      d.nss = (nbmax + L * kRows - 1) / (L * kRows);
+      d.base = base;
+      gtmp[gi] = d;
+      pad += (unsigned long long)128 * d.nss * kRows * (unsigned long long)mpad;
+      // longest groups first over the whole batch (a wave's time is about its
+      // super-strips times its steps per super-strip)
+      const int cb = kCostBinsPerClass - 1 - cost_bin((int64_t)d.nss * (mpad + L - 1));
+      gbin[gi] = cb;
+      grank[gi] = atomicAdd(&s_cost[cb], 1);
     }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    mpad = max(mpad, __shfl_xor(mpad, off));
-    nbmax = max(nbmax, __shfl_xor(nbmax, off));
+  if (lane == 0) s_pad[w] = pad;
+  __syncthreads();
+  for (int i = tid; i < kCostBins; i += blockDim.x) {
+    const int cnt = s_cost[i];
+    if (cnt) s_cost[i] = atomicAdd(&costhist[i], cnt);  // count -> the block's base
   }
-  if (lane != 0) return;
-  GroupDesc d;
-  d.cls = c;
-  d.mpad = mpad;
-  d.nss = (nbmax + L * kRows - 1) / (L * kRows);
-  d.base = base;
-  gtmp[gi] = d;
-  atomicAdd(&ctl->padded_cells, (unsigned long long)128 * d.nss * kRows * (unsigned long long)mpad);
-  // longest groups first over the whole batch (a wave's time is about its
-  // super-strips times its steps per super-strip)
-  const int cb = kCostBinsPerClass - 1 - cost_bin((int64_t)d.nss * (mpad + L - 1));
-  gbin[gi] = cb;
-  atomicAdd(&costhist[cb], 1);
+  if (tid == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < nw; ++i) t += s_pad[i];
+    if (t) atomicAdd(&ctl->padded_cells, t);
+  }
+  __syncthreads();
+  for (int gi = (int)blockIdx.x * nw + w; gi < ng; gi += stride)
+    if (lane == 0) grank[gi] += s_cost[gbin[gi]];
 }
 
 // Single block: exclusive scan of the cost bins.
@@ -727,10 +825,11 @@ __global__ __launch_bounds__(1024) void k_scan_cost(int32_t *costhist) {
 }
 
 __global__ void k_order_groups(int32_t max_groups, const BucketCtl *ctl, const GroupDesc *gtmp,
-                               const int32_t *gbin, int32_t *coff, GroupDesc *groups) {
-  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi >= max_groups || gi >= ctl->ngroups) return;
-  groups[atomicAdd(&coff[gbin[gi]], 1)] = gtmp[gi];
+                               const int32_t *gbin, const int32_t *grank, const int32_t *coff,
+                               GroupDesc *groups) {
+  const int ng = min(max_groups, ctl->ngroups);
+  for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < ng; gi += gridDim.x * blockDim.x)
+    groups[coff[gbin[gi]] + grank[gi]] = gtmp[gi];
 }
 
 // ---------------------------------------------------------------------------
@@ -859,9 +958,9 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
   const int n = B.n;
   const int nb = nblocks(std::max(n, 1), 256);
   hipLaunchKernelGGL(k_sum_cells, dim3(nb), dim3(256), 0, st, n, B.a_len, B.b_len, B.ctl);
-  hipLaunchKernelGGL(k_classify, dim3(nb), dim3(256), 0, st, n, B.a_len, B.b_len,
-                     B.target_slots, B.plain ? 0 : B.force_class, B.bin_of, B.hist, B.ctl,
-                     B.scores, B.a_range, B.b_range);
+  hipLaunchKernelGGL(k_classify, dim3(nblocks(std::max(n, 1), kClsThreads)), dim3(kClsThreads), 0,
+                     st, n, B.a_len, B.b_len, B.target_slots, B.plain ? 0 : B.force_class,
+                     B.bin_of, B.rank, B.hist, B.ctl, B.scores, B.a_range, B.b_range);
   if (B.plain) {
     hipLaunchKernelGGL(k_plain_setup, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.sorted);
     hipLaunchKernelGGL(k_plain_groups, dim3(nblocks((int64_t)n * 64 + 64, 256)), dim3(256), 0,
@@ -869,13 +968,14 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(kScanThreads), 0, st, B.hist, B.ctl);
-  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.hist, B.sorted);
-  const int gb = nblocks(B.max_groups, 256);
-  hipLaunchKernelGGL(k_groups, dim3(nblocks((int64_t)B.max_groups * 64, 256)), dim3(256), 0, st,
-                     B.max_groups, B.sorted, B.a_len, B.b_len, B.ctl, B.gtmp, B.gbin, B.costhist);
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.rank, B.hist,
+                     B.sorted);
+  const int gb = std::min(nblocks(B.max_groups, kGroupThreads / 64), kGroupBlocks);
+  hipLaunchKernelGGL(k_groups, dim3(gb), dim3(kGroupThreads), 0, st, B.max_groups, B.sorted,
+                     B.a_len, B.b_len, B.ctl, B.gtmp, B.gbin, B.grank, B.costhist);
   hipLaunchKernelGGL(k_scan_cost, dim3(1), dim3(1024), 0, st, B.costhist);
-  hipLaunchKernelGGL(k_order_groups, dim3(gb), dim3(256), 0, st, B.max_groups, B.ctl, B.gtmp,
-                     B.gbin, B.costhist, B.groups);
+  hipLaunchKernelGGL(k_order_groups, dim3(std::min(nblocks(B.max_groups, 256), 512)), dim3(256), 0,
+                     st, B.max_groups, B.ctl, B.gtmp, B.gbin, B.grank, B.costhist, B.groups);
   return hipGetLastError();
 }
 

@@ -36,6 +36,9 @@ struct dim3 {
 struct uint2 {
   uint32_t x, y;
 };
+struct int4 {
+  int x, y, z, w;
+};
 inline uint2 make_uint2(uint32_t a, uint32_t b) { return uint2{a, b}; }
 
 namespace emu {
@@ -160,6 +163,10 @@ inline unsigned long long atomicAdd(unsigned long long *p, unsigned long long v)
   return std::atomic_ref<unsigned long long>(*p).fetch_add(v);
 }
 inline int atomicOr(int *p, int v) { return std::atomic_ref<int>(*p).fetch_or(v); }
+inline int atomicCAS(int *p, int cmp, int v) {
+  std::atomic_ref<int>(*p).compare_exchange_strong(cmp, v);
+  return cmp;  // the value found
+}
 inline int atomicMax(int *p, int v) {
   std::atomic_ref<int> a(*p);
   int cur = a.load();
