@@ -32,6 +32,12 @@ constexpr int kSortedPad = kRuns * kRunAlign;  // empty slots the run alignment 
 constexpr int kCostBinsPerClass = 256;
 constexpr int kCostBins = 2048;  // >= kCostBinsPerClass, multiple of 1024
 static_assert(kCostBins >= kCostBinsPerClass, "cost bins");
+// Band groups: a whole-wave pipeline group (L = 64) with at least two
+// super-strips is shared by the waves of one block, one super-strip per wave
+// at a time; a block keeps up to one band group per wave in flight, each with
+// its own boundary row of kWaveScratchWords words.
+constexpr int kBandSlots = kFastThreads / 64;
+constexpr int kMaxStrips = 64;  // super-strips of 1,024 rows cover 65,535 rows
 // Saturation value of the packed int16 path.
 constexpr int32_t kSat16 = 32767;
 
@@ -54,7 +60,9 @@ struct BucketCtl {
   int32_t ovf_count;   // int16-saturated comparisons listed for int32
   int32_t maxm;        // longest A of the batch
   int32_t maxn;        // longest B of the batch
-  int32_t queue;       // next group of the packed kernel's work queue
+  int32_t queue;       // next per-wave group of the packed kernel's work queue
+  int32_t nband;       // band groups: the first nband entries of the group order
+  int32_t band_queue;  // next band group (taken by a whole block)
   int32_t pad0;
   unsigned long long total_cells;
   unsigned long long padded_cells;
@@ -73,6 +81,7 @@ struct FastArgs {
   const GroupDesc *groups;     // in processing order
   BucketCtl *ctl;              // group ranges per class, error bits
   uint32_t *scratch;           // kWaveScratchWords per wave
+  uint32_t *band_scratch;      // kBandSlots * kWaveScratchWords per block
   int32_t *scores;
   int32_t *a_range;
   int32_t *b_range;

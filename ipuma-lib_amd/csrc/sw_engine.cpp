@@ -109,7 +109,7 @@ struct ipuma_engine {
   int generic_blocks = 512;
   int64_t target_slots = 0;
   int rounds = 1;  // groups per resident wave the bucketing aims for
-  DevBuf scratch, work;
+  DevBuf scratch, band_scratch, work;
   ChainBufs fwd, rev;
   DevBuf rcodes, rlen, rres;
   DevBuf s_pairs, s_res;  // one-vs-many search: pair descriptors and results
@@ -198,6 +198,8 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   }
   const size_t scratch_bytes = (size_t)e->fast_blocks * (kFastThreads / 64) * kWaveScratchWords * 4;
   if ((rc = e->scratch.ensure(scratch_bytes))) return rc;
+  if ((rc = e->band_scratch.ensure((size_t)e->fast_blocks * kBandSlots * kWaveScratchWords * 4)))
+    return rc;
   if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
   HIP_TRY(launch_fill_i32(c.hist.as<int32_t>(), kNumBins, 0, st));
   HIP_TRY(launch_fill_i32(c.costhist.as<int32_t>(), kCostBins, 0, st));
@@ -214,7 +216,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   FastArgs A{};
   A.codes = codes; A.a_off = a_off; A.a_len = a_len; A.b_off = b_off; A.b_len = b_len;
   A.table = e->d_table; A.sorted = c.sorted.as<int32_t>(); A.groups = c.groups.as<GroupDesc>();
-  A.ctl = ctl; A.scratch = e->scratch.as<uint32_t>();
+  A.ctl = ctl; A.scratch = e->scratch.as<uint32_t>(); A.band_scratch = e->band_scratch.as<uint32_t>();
   A.scores = scores; A.a_range = a_range; A.b_range = b_range;
   A.ovf_list = c.ovf.as<int32_t>(); A.ovf_count = &ctl->ovf_count;
   A.max_groups = n + kRuns;
@@ -307,6 +309,7 @@ int check_device_error(ipuma_engine *e) {
   if (err & 1) return fail(IPUMA_ETOOLONG, "a sequence is longer than %d", kMaxSeqLen);
   if (err & 2) return fail(IPUMA_EDEVICE, "internal: an end position outside its sequence");
   if (err & 4) return fail(IPUMA_EDEVICE, "internal: a work-group descriptor out of range");
+  if (err & 8) return fail(IPUMA_EDEVICE, "internal: no free band slot");
   return IPUMA_OK;
 }
 
@@ -426,6 +429,7 @@ void ipuma_engine_destroy(ipuma_engine *e) {
   if (!e) return;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   e->scratch.release();
+  e->band_scratch.release();
   e->work.release();
   e->fwd.release();
   e->rev.release();
@@ -643,6 +647,7 @@ int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out) {
     if (c.class_count[k] > 0) lmax = kClassL[k];
   out->lanes_per_pair_max = lmax;
   out->rows_per_lane = kRows;
+  out->band_groups = c.nband;
   if (e->ev_recorded) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, e->ev[1], e->ev[2]));

@@ -333,6 +333,361 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   }
 }
 
+// ---------------------------------------------------------------------------
+// Band groups: whole-wave pipeline groups (L = 64) with two or more
+// super-strips, shared by the waves of one block.  Super-strip s needs the
+// boundary row that super-strip s-1 writes; the group's boundary buffer is
+// updated in place (strip s reads column j at its step j - 1 and overwrites it
+// at step j + 63), and the writer of strip s-1 publishes how many columns it
+// has written in LDS (prog[s-1]), which strip s waits for before reading.
+// Strips are handed out in order, so a strip only waits for one handed out
+// earlier, which itself only waits for an earlier one: every wait ends.  A
+// block holds at most one band group per wave at a time (one slot each, each
+// slot with its own boundary buffer); the wave that finds the current group's
+// strips exhausted loads the next band group into a free slot.
+//
+// Ticket, CAS and counter updates run on every lane with lane 0 carrying the
+// update and the other lanes a no-op (CAS with an impossible expected value,
+// add of 0), read back with readfirstlane: no lane-conditional branch
+// surrounds them (see next_group).
+// ---------------------------------------------------------------------------
+struct BandSlot {
+  GroupDesc gd;
+  int32_t ticket;  // seq << 16 | nss << 8 | next strip; next == nss + 1: loading the next group
+  int32_t done;    // strips finished
+  int32_t prog[kMaxStrips];      // boundary columns each strip has written
+  unsigned long long best[2];    // H << 32 | (0xFFFF - row) << 16 | column (low, high)
+};
+struct BandState {
+  BandSlot slot[kBandSlots + 1];  // [kBandSlots]: the empty group before the first
+  int32_t cur;                    // seq << 4 | slot of the group handing out strips, -1: none left
+  int32_t free_mask;              // slots not holding a group
+};
+static_assert(kBandSlots < 16 && kMaxStrips < 255, "band ticket layout");
+
+// Loads of LDS words other waves update are read back through readfirstlane:
+// the wave's control flow depends on them, so it must see one value.
+__device__ __forceinline__ int lds_load_acq(int *p) {
+  return __builtin_amdgcn_readfirstlane(
+      __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+__device__ __forceinline__ void lds_store_rel(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// Compare-and-swap carried by lane 0; the wave reads lane 0's result.
+__device__ __forceinline__ int lds_cas_lane0(int *p, int expected, int desired, int lane) {
+  int e = lane == 0 ? expected : -1;  // tickets are never -1: the other lanes never write
+  const int d = lane == 0 ? desired : -1;
+  __hip_atomic_compare_exchange_strong(p, &e, d, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+  return __builtin_amdgcn_readfirstlane(e);
+}
+
+// Waits until the producer of the boundary row has written `need` columns.
+__device__ __forceinline__ int band_wait(int *prog, int need) {
+  int v = lds_load_acq(prog);
+  while (v < need) {
+    __builtin_amdgcn_s_sleep(1);
+    v = lds_load_acq(prog);
+  }
+  return v;
+}
+
+// Super-strip s of the band group in slot `sl`: the whole-wave pipeline body
+// of run_group, with the boundary row shared through the slot's buffer.
+template <int KP>
+__device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
+                           int sl, int s, int lane) {
+  constexpr int R = kRows;
+  constexpr uint32_t PADC = KP - 1;
+  constexpr uint32_t COLSTRIDE = KP * 128;
+  BandSlot &S = B.slot[sl];
+  const int k = lane;
+  const bool first = (k == 0);
+  const bool last = (k == 63);
+  const int mpad = __builtin_amdgcn_readfirstlane(S.gd.mpad);
+  const int nss = __builtin_amdgcn_readfirstlane(S.gd.nss);
+  const int gbase = __builtin_amdgcn_readfirstlane(S.gd.base);
+  const uint32_t bank4 = (uint32_t)(lane & 31) * 4u;
+  uint32_t *__restrict__ bscr =
+      A.band_scratch + ((int64_t)blockIdx.x * kBandSlots + sl) * kWaveScratchWords;
+
+  const int p_lo = A.sorted[gbase];
+  const int p_hi = A.sorted[gbase + 1];
+  int m_lo = 0, n_lo = 0, m_hi = 0, n_hi = 0;
+  const uint8_t *a_lo = A.codes, *b_lo = A.codes, *a_hi = A.codes, *b_hi = A.codes;
+  if (p_lo >= 0) {
+    m_lo = A.a_len[p_lo];
+    n_lo = A.b_len[p_lo];
+    a_lo = A.codes + A.a_off[p_lo];
+    b_lo = A.codes + A.b_off[p_lo];
+  }
+  if (p_hi >= 0) {
+    m_hi = A.a_len[p_hi];
+    n_hi = A.b_len[p_hi];
+    a_hi = A.codes + A.a_off[p_hi];
+    b_hi = A.codes + A.b_off[p_hi];
+  }
+  const uint32_t *a32_lo = (const uint32_t *)a_lo;
+  const uint32_t *a32_hi = (const uint32_t *)a_hi;
+  const uint32_t gap_o = A.gap_o, gap_x = A.gap_x;
+
+  const int row0 = ((s << 6) + k) * R;
+  uint32_t rp[R];
+  {
+    uint32_t wl[R / 4], wh[R / 4];
+    const uint32_t *bl = (const uint32_t *)(b_lo + row0);
+    const uint32_t *bh = (const uint32_t *)(b_hi + row0);
+#pragma unroll
+    for (int q = 0; q < R / 4; ++q) {
+      wl[q] = (row0 + 4 * q < n_lo) ? bl[q] : 0u;
+      wh[q] = (row0 + 4 * q < n_hi) ? bh[q] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t cl =
+          (row0 + r < n_lo) ? min((wl[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
+      const uint32_t ch =
+          (row0 + r < n_hi) ? min((wh[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
+      rp[r] = (cl * 128u) | ((ch * 128u) << 16);
+    }
+  }
+  uint32_t H[R], E[R], KL[R], KH[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    H[r] = 0u;
+    E[r] = 0u;
+    KL[r] = 0u;
+    KH[r] = 0u;
+  }
+  uint32_t out_h = 0u, out_f = 0u, hdiag = 0u;
+  uint32_t col_lo = PADC * COLSTRIDE, col_hi = PADC * COLSTRIDE;
+  const bool load_bnd = s > 0;
+  const bool store_bnd = s + 1 < nss;
+  int *const prev_prog = &S.prog[s > 0 ? s - 1 : 0];
+  int *const my_prog = &S.prog[s];
+  int avail = 0;
+
+  uint2 nb = make_uint2(0u, 0u);
+  if (load_bnd && mpad > 0) {
+    avail = band_wait(prev_prog, 1);
+    if (first) nb = *(const uint2 *)bscr;
+  }
+  uint32_t na_lo = 0u, na_hi = 0u;
+  if (first) {
+    na_lo = (0 < m_lo) ? a32_lo[0] : 0u;
+    na_hi = (0 < m_hi) ? a32_hi[0] : 0u;
+  }
+  const int T = mpad + 63;
+  for (int t0 = 0; t0 < T; t0 += 4) {
+    // columns t0 + 1 .. t0 + 4 of the boundary row are read below
+    if (load_bnd) {
+      const int need = min(mpad, t0 + 5);
+      if (avail < need) avail = band_wait(prev_prog, need);
+    }
+    const uint32_t wa_lo = na_lo, wa_hi = na_hi;
+    if (first) {
+      const int tn = t0 + 4;
+      na_lo = (tn < m_lo) ? a32_lo[tn >> 2] : 0u;
+      na_hi = (tn < m_hi) ? a32_hi[tn >> 2] : 0u;
+    }
+#pragma unroll 1
+    for (int u = 0; u < 4; ++u) {
+      const int t = t0 + u;
+      const uint2 cb = nb;
+      if (first && load_bnd && t + 1 < mpad)
+        nb = *(const uint2 *)(bscr + (size_t)(t + 1) * 2);
+      else
+        nb = make_uint2(0u, 0u);
+      const uint32_t ac_lo = (t < m_lo) ? min((wa_lo >> (8 * u)) & 0xffu, PADC) : PADC;
+      const uint32_t ac_hi = (t < m_hi) ? min((wa_hi >> (8 * u)) & 0xffu, PADC) : PADC;
+      const uint32_t rh = from_prev<kModeWave>(cb.x, out_h, first);
+      const uint32_t rf = from_prev<kModeWave>(cb.y, out_f, first);
+      col_lo = from_prev<kModeWave>(ac_lo * COLSTRIDE, col_lo, first);
+      col_hi = from_prev<kModeWave>(ac_hi * COLSTRIDE, col_hi, first);
+      const int j = t - k;
+      const uint32_t jinv = 0xFFFFu - (uint32_t)j;
+      const char *tl = (const char *)tab + (col_lo + bank4);
+      const char *th = (const char *)tab + (col_hi + bank4);
+#pragma unroll
+      for (int r = 0; r < R; ++r) asm volatile("" : "+v"(rp[r]));
+      uint32_t sv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint32_t slo = *(const uint16_t *)(tl + (rp[r] & 0xFFFFu));
+        const uint32_t shi = *(const uint16_t *)(th + (rp[r] >> 16));
+        sv[r] = __builtin_amdgcn_perm(shi, slo, 0x05040100u);
+      }
+      s16x2 t2 = __builtin_elementwise_add_sat(S2(hdiag), S2(sv[0]));
+      hdiag = rh;
+      uint32_t f = rf;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        s16x2 t2n = t2;
+        if (r + 1 < R) t2n = __builtin_elementwise_add_sat(S2(H[r]), S2(sv[r + 1]));
+        const uint32_t hn =
+            W(__builtin_elementwise_max(__builtin_elementwise_max(t2, S2(E[r])), S2(f)));
+        H[r] = hn;
+        const u16x2 ho = __builtin_elementwise_sub_sat(U2(hn), U2(gap_o));
+        E[r] = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))));
+        f = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(f), U2(gap_x))));
+        KL[r] = max(KL[r], __builtin_amdgcn_perm(hn, jinv, 0x05040100u));
+        KH[r] = max(KH[r], __builtin_amdgcn_perm(hn, jinv, 0x07060100u));
+        t2 = t2n;
+      }
+      out_h = H[R - 1];
+      out_f = f;
+      if (last && store_bnd && j >= 0 && j < mpad)
+        *(uint2 *)(bscr + (size_t)j * 2) = make_uint2(out_h, out_f);
+    }
+    // publish the boundary columns written so far (lane 63 wrote j = t - 63)
+    if (store_bnd && (t0 & 15) == 12) {
+      __builtin_amdgcn_wave_barrier();
+      lds_store_rel(my_prog, max(0, min(mpad, t0 - 59)));
+    }
+  }
+  if (store_bnd) {
+    __builtin_amdgcn_wave_barrier();
+    lds_store_rel(my_prog, mpad);
+  }
+
+  int bh_lo = 0, bi_lo = 0, bj_lo = 0, bh_hi = 0, bi_hi = 0, bj_hi = 0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int hl = (int)(KL[q] >> 16);
+    if (hl > bh_lo) {
+      bh_lo = hl;
+      bi_lo = row0 + q;
+      bj_lo = 0xFFFF - (int)(KL[q] & 0xFFFFu);
+    }
+    const int hh = (int)(KH[q] >> 16);
+    if (hh > bh_hi) {
+      bh_hi = hh;
+      bi_hi = row0 + q;
+      bj_hi = 0xFFFF - (int)(KH[q] & 0xFFFFu);
+    }
+  }
+  for (int off = 1; off < 64; off <<= 1) {
+    const int oh = __shfl_xor(bh_lo, off), oi = __shfl_xor(bi_lo, off), oj = __shfl_xor(bj_lo, off);
+    if (oh > bh_lo || (oh == bh_lo && oi < bi_lo)) {
+      bh_lo = oh;
+      bi_lo = oi;
+      bj_lo = oj;
+    }
+    const int qh = __shfl_xor(bh_hi, off), qi = __shfl_xor(bi_hi, off), qj = __shfl_xor(bj_hi, off);
+    if (qh > bh_hi || (qh == bh_hi && qi < bi_hi)) {
+      bh_hi = qh;
+      bi_hi = qi;
+      bj_hi = qj;
+    }
+  }
+  // strips hold disjoint rows: the group's first maximum is (max H, min row)
+  const unsigned long long key_lo = ((unsigned long long)(uint32_t)bh_lo << 32) |
+                                    ((unsigned long long)(0xFFFF - (bi_lo & 0xFFFF)) << 16) |
+                                    (unsigned long long)(bj_lo & 0xFFFF);
+  const unsigned long long key_hi = ((unsigned long long)(uint32_t)bh_hi << 32) |
+                                    ((unsigned long long)(0xFFFF - (bi_hi & 0xFFFF)) << 16) |
+                                    (unsigned long long)(bj_hi & 0xFFFF);
+  __hip_atomic_fetch_max(&S.best[0], key_lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __hip_atomic_fetch_max(&S.best[1], key_hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  const int d = __builtin_amdgcn_readfirstlane(__hip_atomic_fetch_add(
+      &S.done, lane == 0 ? 1 : 0, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP));
+  if (d != nss - 1) return;
+  // the last strip of the group: write its results and free the slot
+  const unsigned long long bl =
+      __hip_atomic_load(&S.best[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  const unsigned long long bhk =
+      __hip_atomic_load(&S.best[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (first) {
+    const int sl_h = (int)(bl >> 32), sl_i = 0xFFFF - (int)((bl >> 16) & 0xFFFF),
+              sl_j = (int)(bl & 0xFFFF);
+    const int sh_h = (int)(bhk >> 32), sh_i = 0xFFFF - (int)((bhk >> 16) & 0xFFFF),
+              sh_j = (int)(bhk & 0xFFFF);
+    if (p_lo >= 0) {
+      if (sl_h >= kSat16) {
+        A.ovf_list[atomicAdd(A.ovf_count, 1)] = p_lo;
+      } else {
+        A.scores[p_lo] = sl_h;
+        A.a_range[p_lo] = (int32_t)((uint32_t)sl_j << 16);
+        A.b_range[p_lo] = (int32_t)((uint32_t)sl_i << 16);
+      }
+    }
+    if (p_hi >= 0) {
+      if (sh_h >= kSat16) {
+        A.ovf_list[atomicAdd(A.ovf_count, 1)] = p_hi;
+      } else {
+        A.scores[p_hi] = sh_h;
+        A.a_range[p_hi] = (int32_t)((uint32_t)sh_j << 16);
+        A.b_range[p_hi] = (int32_t)((uint32_t)sh_i << 16);
+      }
+    }
+  }
+  __hip_atomic_fetch_or(&B.free_mask, 1 << sl, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Loads the next band group into a free slot and makes it current (called by
+// the wave that exhausted group `cur`), or marks the band phase finished.
+__device__ void band_load(const FastArgs &A, BandState &B, int cur, int lane, int nband) {
+  const int seq = ((cur >> 4) + 1) & 0xFFFF;
+  for (;;) {
+    const int tk = __builtin_amdgcn_readfirstlane(atomicAdd(&A.ctl->band_queue, lane == 0 ? 1 : 0));
+    if (tk >= nband) break;
+    const GroupDesc gd = A.groups[tk];
+    const bool ok = gd.cls == kNumClasses - 1 && gd.base >= 0 && gd.mpad > 0 &&
+                    gd.mpad <= kMaxSeqLen && gd.nss >= 2 && gd.nss <= kMaxStrips;
+    if (!ok) {
+      atomicOr(&A.ctl->error, 4);
+      continue;
+    }
+    const int mask = lds_load_acq(&B.free_mask);
+    if (mask == 0) {  // cannot happen: at most kBandSlots - 1 other groups are unfinished
+      atomicOr(&A.ctl->error, 8);
+      break;
+    }
+    const int sl = __builtin_ctz((unsigned)mask);
+    __hip_atomic_fetch_and(&B.free_mask, ~(1 << sl), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    BandSlot &S = B.slot[sl];
+    S.gd = gd;
+    S.done = 0;
+    for (int i = lane; i < kMaxStrips; i += 64) S.prog[i] = 0;
+    S.best[0] = 0ull;
+    S.best[1] = 0ull;
+    __builtin_amdgcn_wave_barrier();
+    // a free slot's ticket is not updated by other waves (they only take
+    // strips of the current group), nor is cur (only its loader moves it)
+    lds_cas_lane0(&S.ticket, lds_load_acq(&S.ticket), (seq << 16) | (gd.nss << 8), lane);
+    lds_cas_lane0(&B.cur, cur, (seq << 4) | sl, lane);
+    return;
+  }
+  lds_cas_lane0(&B.cur, cur, -1, lane);
+}
+
+// The band phase of a block: its waves take strips of the current band group
+// until no band group is left.
+template <int KP>
+__device__ void band_phase(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
+                           int lane, int nband) {
+  for (;;) {
+    const int cur = lds_load_acq(&B.cur);
+    if (cur < 0) return;
+    const int sl = cur & 15, seq = (cur >> 4) & 0xFFFF;
+    BandSlot &S = B.slot[sl];
+    const int t = lds_load_acq(&S.ticket);
+    if (((t >> 16) & 0xFFFF) != seq) {  // the slot is being reloaded: re-read cur
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    const int nss = (t >> 8) & 0xFF, nx = t & 0xFF;
+    if (nx <= nss) {
+      if (lds_cas_lane0(&S.ticket, t, t + 1, lane) != t) continue;  // another wave took it
+      if (nx < nss) band_strip<KP>(A, tab, B, sl, nx, lane);
+      else band_load(A, B, cur, lane, nband);
+      continue;
+    }
+    while (lds_load_acq(&B.cur) == cur) __builtin_amdgcn_s_sleep(2);
+  }
+}
+
 // Next group for the calling wave, or -1 when the queue is empty.  Every
 // lane executes the same atomic (lane 0 adds 1, the others 0), so no
 // lane-conditional branch surrounds it; the wave takes lane 0's ticket.  (A
@@ -340,8 +695,8 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
 // unsafe here: for the 64-lane pipeline that branch has the same predicate as
 // the first lane's result write, the compiler may thread one into the other
 // across the loop back-edge, and lanes 1..63 then loop without lane 0.)
-__device__ __forceinline__ int next_group(int32_t *queue, int ng, int lane) {
-  const int t = __builtin_amdgcn_readfirstlane(atomicAdd(queue, lane == 0 ? 1 : 0));
+__device__ __forceinline__ int next_group(int32_t *queue, int first, int ng, int lane) {
+  const int t = first + __builtin_amdgcn_readfirstlane(atomicAdd(queue, lane == 0 ? 1 : 0));
   return t < ng ? t : -1;
 }
 
@@ -352,15 +707,26 @@ template <int KP>
 __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   constexpr int WPB = kFastThreads / 64;
   __shared__ uint32_t tab[KP * KP * 32];
+  __shared__ BandState band;
   const int lane = threadIdx.x & 63;
   const int ng = min(A.ctl->ngroups, A.max_groups);
   if (ng <= 0) return;  // uniform over the grid
+  const int nband = min(A.ctl->nband, ng);
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < kBandSlots; ++i) band.slot[i].ticket = 0;
+    band.slot[kBandSlots].ticket = 0;  // seq 0, no strips: the first wave loads a group
+    band.cur = kBandSlots;
+    band.free_mask = (1 << kBandSlots) - 1;
+  }
   for (int i = threadIdx.x; i < KP * KP * 32; i += blockDim.x)
     tab[i] = (uint32_t)(uint16_t)A.table[i >> 5];
   __syncthreads();
+  // band groups first (they are the longest), one block each, then the rest
+  if (nband > 0) band_phase<KP>(A, tab, band, lane, nband);
   const int wave = (int)blockIdx.x * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
-  for (int gi = next_group(&A.ctl->queue, ng, lane); gi >= 0; gi = next_group(&A.ctl->queue, ng, lane)) {
+  for (int gi = next_group(&A.ctl->queue, nband, ng, lane); gi >= 0;
+       gi = next_group(&A.ctl->queue, nband, ng, lane)) {
     const GroupDesc gd = A.groups[gi];
     // a descriptor outside what the bucketing produces (class, scratch row
     // mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and skipped
@@ -739,7 +1105,10 @@ __global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, co
                                                           int32_t *grank, int32_t *costhist) {
   __shared__ int32_t s_cost[kCostBins];
   __shared__ unsigned long long s_pad[kGroupThreads / 64];
+  __shared__ int32_t s_nband;
+  static_assert(kCostBins >= 2 * kCostBinsPerClass, "band and other cost bins");
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid == 0) s_nband = 0;
   for (int i = tid; i < kCostBins; i += blockDim.x) s_cost[i] = 0;
   __syncthreads();
   const int ng = min(max_groups, ctl->ngroups);
@@ -778,9 +1147,14 @@ __global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, co
       pad += (unsigned long long)128 * d.nss * kRows * (unsigned long long)mpad;
       // longest groups first over the whole batch (a wave's time is about its
       // super-strips times its steps per super-strip)
-      const int cb = kCostBinsPerClass - 1 - cost_bin((int64_t)d.nss * (mpad + L - 1));
+      // band groups (whole-wave pipelines over two or more super-strips,
+      // shared by a block) come before all others
+      const bool band = c == kNumClasses - 1 && d.nss >= 2;
+      const int cb = (band ? 0 : kCostBinsPerClass) + kCostBinsPerClass - 1 -
+                     cost_bin((int64_t)d.nss * (mpad + L - 1));
       gbin[gi] = cb;
       grank[gi] = atomicAdd(&s_cost[cb], 1);
+      if (band) atomicAdd(&s_nband, 1);
     }
   }
   if (lane == 0) s_pad[w] = pad;
@@ -793,6 +1167,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, co
     unsigned long long t = 0;
     for (int i = 0; i < nw; ++i) t += s_pad[i];
     if (t) atomicAdd(&ctl->padded_cells, t);
+    if (s_nband) atomicAdd(&ctl->nband, s_nband);
   }
   __syncthreads();
   for (int gi = (int)blockIdx.x * nw + w; gi < ng; gi += stride)

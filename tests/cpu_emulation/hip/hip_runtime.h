@@ -138,6 +138,7 @@ void launch(const char *name, K kernel, dim3 grid, dim3 block, Args... args) {
 #define __builtin_amdgcn_perm(a, b, s) emu::perm((a), (b), (s))
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_readfirstlane(v) emu::exchange_get((v), 0)
+#define __builtin_amdgcn_wave_barrier() ((void)emu::exchange_get(0, 0))
 
 template <class T> inline T __shfl_xor(T v, int m) { return emu::exchange_get(v, emu::ctx.lane ^ m); }
 template <class T> inline T __shfl(T v, int src) { return emu::exchange_get(v, src); }
@@ -173,6 +174,34 @@ inline int atomicMax(int *p, int v) {
   while (cur < v && !a.compare_exchange_weak(cur, v)) {}
   return cur;
 }
+
+// scoped atomics (every scope is the whole emulated device)
+namespace emu {
+template <class T> inline T hload(T *p) { return std::atomic_ref<T>(*p).load(); }
+template <class T, class V> inline void hstore(T *p, V v) { std::atomic_ref<T>(*p).store((T)v); }
+template <class T, class V> inline bool hcas(T *p, T *e, V d) {
+  return std::atomic_ref<T>(*p).compare_exchange_strong(*e, (T)d);
+}
+template <class T, class V> inline T hadd(T *p, V v) { return std::atomic_ref<T>(*p).fetch_add((T)v); }
+template <class T, class V> inline T hor(T *p, V v) { return std::atomic_ref<T>(*p).fetch_or((T)v); }
+template <class T, class V> inline T hand(T *p, V v) { return std::atomic_ref<T>(*p).fetch_and((T)v); }
+template <class T, class V> inline T hmax(T *p, V v) {
+  std::atomic_ref<T> a(*p);
+  T cur = a.load();
+  while (cur < (T)v && !a.compare_exchange_weak(cur, (T)v)) {}
+  return cur;
+}
+}  // namespace emu
+#define __HIP_MEMORY_SCOPE_WORKGROUP 3
+#define __HIP_MEMORY_SCOPE_AGENT 4
+#define __hip_atomic_load(p, o, s) emu::hload(p)
+#define __hip_atomic_store(p, v, o, s) emu::hstore((p), (v))
+#define __hip_atomic_compare_exchange_strong(p, e, d, o1, o2, s) emu::hcas((p), (e), (d))
+#define __hip_atomic_fetch_add(p, v, o, s) emu::hadd((p), (v))
+#define __hip_atomic_fetch_or(p, v, o, s) emu::hor((p), (v))
+#define __hip_atomic_fetch_and(p, v, o, s) emu::hand((p), (v))
+#define __hip_atomic_fetch_max(p, v, o, s) emu::hmax((p), (v))
+#define __builtin_amdgcn_s_sleep(n) std::this_thread::yield()
 
 inline int max(int a, int b) { return a > b ? a : b; }
 inline int min(int a, int b) { return a < b ? a : b; }

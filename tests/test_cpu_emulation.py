@@ -188,3 +188,34 @@ def test_search_topk_emulated(emu_lib):
         assert np.array_equal(ti.reshape(3, k), ri), k
         assert np.array_equal(ts.reshape(3, k), rs), k
     eng.close()
+
+
+def test_band_groups_emulated(emu_lib):
+    """Whole-wave groups of two or more super-strips (|B| > 1024) shared by the
+    waves of a block: long DNA pairs among short ones, scores and both
+    ranges (the start pass runs the band groups again on the reversed
+    prefixes), against the oracle."""
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(21)
+    A, B = [], []
+    for i in range(8):
+        m = int(rng.integers(200, 700))
+        a = "".join(rng.choice(list("ACGT"), m))
+        mut = "".join(c if rng.random() > 0.1 else str(rng.choice(list("ACGT"))) for c in a)
+        left = "".join(rng.choice(list("ACGT"), int(rng.integers(400, 1500))))
+        right = "".join(rng.choice(list("ACGT"), int(rng.integers(100, 600))))
+        A.append(a)
+        B.append(left + mut + right)
+    for i in range(24):
+        A.append("".join(rng.choice(list("ACGT"), int(rng.integers(5, 60)))))
+        B.append("".join(rng.choice(list("ACGT"), int(rng.integers(5, 60)))))
+    eng = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    s, ar, br = eng.compare(A, B)
+    st = eng.stats()
+    assert st["band_groups"] >= 2, st
+    exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1, forward_only=False)
+    for got, want in zip((s, ar, br), exp):
+        assert np.array_equal(np.asarray(got), np.asarray(want))
+    eng.close()

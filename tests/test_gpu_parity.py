@@ -206,6 +206,32 @@ def test_long_pairs(gpu):
     assert_same(device_run(eng, batch), oracle_batch(batch, False), "long")
 
 
+def test_band_groups(gpu):
+    # whole-wave groups of two or more super-strips are shared by the waves of
+    # a block: thousands of them (several per block, so the block's group
+    # slots are reused) and a few very long pairs, with starts
+    note("band groups: 8000 pairs |B| 1-2 kb + 40 pairs 1.5-12 kb, with starts")
+    rng = np.random.default_rng(17)
+    pairs = []
+    for _ in range(8000):
+        a = W._dna_draw(rng, int(rng.integers(40, 120)))
+        left = W._dna_draw(rng, int(rng.integers(900, 2000)))
+        pairs.append((a, np.concatenate([left, W._mutate(rng, a, 0.1, 0.02, W._dna_draw,
+                                                         W._dna_subst)])))
+    for _ in range(40):
+        a = W._dna_draw(rng, int(rng.integers(1500, 12000)))
+        pairs.append((a, W._mutate(rng, a, 0.15, 0.02, W._dna_draw, W._dna_subst)))
+    batch = W.make_batch(pairs, "band", W.DNA_SW)
+    exp = oracle_batch(batch, False)
+    for lanes in (0, 64):  # the classifier's choice, then every pair on 64 lanes
+        eng = engine_for(batch["sw"], False)
+        if lanes:
+            eng.set_lanes_per_pair(lanes)
+        got = device_run(eng, batch)
+        assert eng.stats()["band_groups"] >= (3500 if lanes else 2000), eng.stats()
+        assert_same(got, exp, f"band groups, lanes={lanes}")
+
+
 def test_int16_saturation_rerun(gpu):
     # W^L vs W^L under BLOSUM62 scores 11*L: 45056 at L = 4096 overflows int16
     note("int16 saturation -> int32 re-run (W^4096)")
