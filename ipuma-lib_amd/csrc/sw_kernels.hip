@@ -36,7 +36,7 @@ __device__ __forceinline__ uint32_t W(s16x2 v) { return __builtin_bit_cast(uint3
 
 // Pipeline modes of the packed kernel: one lane per comparison (L = 1, no
 // cross-lane moves), 2..16 lanes (DPP row_shr:1 inside rows of 16 lanes), and
-// the whole wave (L = 64, moves by shuffle).
+// the whole wave (L = 64, DPP wave_shr:1).
 constexpr int kModeSolo = 0;
 constexpr int kModeRow = 1;
 constexpr int kModeWave = 2;
@@ -49,7 +49,8 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
     (void)v; (void)first;
     return own;
   } else if constexpr (MODE == kModeWave) {
-    const uint32_t r = (uint32_t)__shfl_up((int)v, 1, 64);
+    // DPP wave_shr:1: lane i takes lane i-1 across the whole wave
+    const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x138, 0xf, 0xf, false);
     return first ? own : r;
   } else {
     const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x111, 0xf, 0xf, false);
@@ -364,6 +365,9 @@ struct BandState {
   int32_t free_mask;              // slots not holding a group
 };
 static_assert(kBandSlots < 16 && kMaxStrips < 255, "band ticket layout");
+// Band groups of at least this many super-strips run their strips at raised
+// wave priority.
+constexpr int kBandPrioStrips = 8;
 
 // Loads of LDS words other waves update are read back through readfirstlane:
 // the wave's control flow depends on them, so it must see one value.
@@ -680,8 +684,15 @@ __device__ void band_phase(const FastArgs &A, const uint32_t *__restrict__ tab, 
     const int nss = (t >> 8) & 0xFF, nx = t & 0xFF;
     if (nx <= nss) {
       if (lds_cas_lane0(&S.ticket, t, t + 1, lane) != t) continue;  // another wave took it
-      if (nx < nss) band_strip<KP>(A, tab, B, sl, nx, lane);
-      else band_load(A, B, cur, lane, nband);
+      if (nx < nss) {
+        // The longest band groups bound the launch: their strips issue ahead
+        // of the other waves on the SIMD (the other waves fill the gaps).
+        if (nss >= kBandPrioStrips) __builtin_amdgcn_s_setprio(2);
+        band_strip<KP>(A, tab, B, sl, nx, lane);
+        __builtin_amdgcn_s_setprio(0);
+      } else {
+        band_load(A, B, cur, lane, nband);
+      }
       continue;
     }
     while (lds_load_acq(&B.cur) == cur) __builtin_amdgcn_s_sleep(2);
@@ -966,12 +977,21 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
         // (cells / (kRows * slots)); the wave scratch sets a floor
         // (mpad * 64/L <= 65536).  Groups of similar length then keep the
         // longest-first queue's tail short.
+        //
+        // Two refinements: a comparison that spans more than one row of 16
+        // never runs alone in a lane (the one-lane pipeline reads and writes
+        // the boundary row on every lane and step; two lanes halve that),
+        // and one that a band group could take (|B| > 1024) must fit a third
+        // of the cap on one wave: otherwise it joins the band groups, which
+        // spread it over the waves of a block, instead of running long after
+        // the band groups as the launch's tail.
         const int64_t cap = max((int64_t)1, (int64_t)(ctl->total_cells /
                                                       (unsigned long long)(kRows * target_slots)));
-        cls = pow2_class((m + 1023) / 1024);
+        const int64_t capc = nb > 64 * kRows ? max((int64_t)1, cap / 3) : cap;
+        cls = max(nb > kRows ? 1 : 0, pow2_class((m + 1023) / 1024));
         while (cls < kNumClasses - 1) {
           const int Lc = kClassL[cls];
-          if ((int64_t)((nb + Lc * kRows - 1) / (Lc * kRows)) * (m + Lc - 1) <= cap) break;
+          if ((int64_t)((nb + Lc * kRows - 1) / (Lc * kRows)) * (m + Lc - 1) <= capc) break;
           ++cls;
         }
       }

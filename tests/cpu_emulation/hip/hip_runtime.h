@@ -202,6 +202,7 @@ template <class T, class V> inline T hmax(T *p, V v) {
 #define __hip_atomic_fetch_and(p, v, o, s) emu::hand((p), (v))
 #define __hip_atomic_fetch_max(p, v, o, s) emu::hmax((p), (v))
 #define __builtin_amdgcn_s_sleep(n) std::this_thread::yield()
+#define __builtin_amdgcn_s_setprio(n) ((void)0)
 
 inline int max(int a, int b) { return a > b ? a : b; }
 inline int min(int a, int b) { return a < b ? a : b; }
