@@ -152,6 +152,56 @@ int ipuma_search_device(ipuma_engine *e, const uint8_t *codes, int64_t codes_len
                         const int64_t *db_off, const int32_t *db_len, int32_t k,
                         int32_t *scores, int32_t *top_idx, int32_t *top_score, void *stream);
 
+/*
+ * Prepared input buffers: the reference's wire format, byte for byte
+ * (src/ipu_batch_affine.cpp:752-950, src/ipu_config.cpp:6-14).  An input
+ * buffer holds tiles_used tile regions of one-byte residue codes, then per
+ * tile max_batches x {|A|, A offset, |B|, B offset} int32 metadata (offsets
+ * inside the tile's region); comparison k lands in slot mapping[k] = tile *
+ * max_batches + position.  The results buffer is [scores | a ranges |
+ * b ranges], one int32 per slot.
+ */
+/* IPUAlgoConfig::getInputBufferSize32b (src/ipu_config.cpp:14): words of an
+ * input buffer for this config (negative on error). */
+int64_t ipuma_input_buffer_words(const ipuma_algo_config *algo);
+/* Replaces SWAlgorithm::prepare_remote (src/ipu_batch_affine.h:114,
+ * .cpp:891-950): the algo config's fill algorithm places comparisons k = A_k
+ * vs B_k (strings as in ipuma_compare_local), the buffer is cleared and
+ * written, mapping[k] receives the slot of comparison k and *max_bucket (may
+ * be NULL) the largest tile fill in bytes.  Host only. */
+int ipuma_prepare_remote(const ipuma_sw_config *sw, const ipuma_algo_config *algo, int32_t n,
+                         const char *a_chars, const int64_t *a_offsets, const char *b_chars,
+                         const int64_t *b_offsets, int32_t *inputs, int64_t input_words,
+                         int32_t *mapping, int32_t *max_bucket);
+/* The preparation step of compare_mn_local (fillMNBuckets + fill_input_buffer,
+ * src/ipu_batch_affine.cpp:697-705, 752-822): comparisons cmps[2k] vs
+ * cmps[2k+1] over one sequence list, a sequence stored once per tile.  The
+ * buffer is cleared first.  Host only. */
+int ipuma_prepare_remote_mn(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
+                            int32_t nseqs, const char *chars, const int64_t *offsets,
+                            int32_t ncmps, const int32_t *cmps, int32_t *inputs,
+                            int64_t input_words, int32_t *mapping, int32_t *max_bucket);
+/* Replaces SWAlgorithm::transferResults (src/ipu_batch_affine.h:115-116,
+ * .cpp:674-695): results of comparison k from slot mapping[k] of each third
+ * of the results buffer (result_words must be a multiple of 3).  Host only. */
+int ipuma_transfer_results(const int32_t *results, int64_t result_words, const int32_t *mapping,
+                           int32_t n, int32_t *scores, int32_t *a_range, int32_t *b_range);
+/* Replaces prepared_remote_compare (src/ipu_batch_affine.h:105,
+ * .cpp:557-562): runs every slot of a prepared buffer (laid out for this
+ * engine's algo config; `lot` is the layout lot of transmission_programs > 1,
+ * else 0) and writes the first result_words words of [scores | a ranges |
+ * b ranges].  Empty slots score 0.  Blocks until the results are written. */
+int ipuma_prepared_compare(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
+                           int32_t *results, int64_t result_words, int32_t lot);
+/* Replaces async_submit_prepared_remote_compare /
+ * blocking_join_prepared_remote_compare (src/ipu_batch_affine.h:107-108,
+ * .cpp:496-555): the engine's executor thread runs submitted buffers in
+ * order (layout lot 0); both buffers must stay valid until ipuma_prepared_join
+ * returns, which reports the job's status.  Any thread may submit or join. */
+int ipuma_prepared_submit(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
+                          int32_t *results, int64_t result_words, int64_t *job);
+int ipuma_prepared_join(ipuma_engine *e, int64_t job);
+
 /* Batching policy of the device path (replaces IPUAlgoConfig.fillAlgo's
  * partition::fillFirst/roundRobin/greedy, src/partition.cpp:197-462). */
 enum {

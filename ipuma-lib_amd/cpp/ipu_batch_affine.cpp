@@ -52,7 +52,8 @@ void pack(const std::vector<std::string> &v, std::string &chars, std::vector<int
 SWAlgorithm::SWAlgorithm(SWConfig config, IPUAlgoConfig algoconfig)
     : SWAlgorithm(config, algoconfig, 0) {}
 
-SWAlgorithm::SWAlgorithm(SWConfig config, IPUAlgoConfig algoconfig, int, size_t, size_t, bool)
+SWAlgorithm::SWAlgorithm(SWConfig config, IPUAlgoConfig algoconfig, int, size_t slotCap, size_t,
+                         bool)
     : algoconfig(algoconfig), config_(config) {
   const ipuma_sw_config s = to_c(config);
   const ipuma_algo_config a = to_c(algoconfig);
@@ -61,11 +62,17 @@ SWAlgorithm::SWAlgorithm(SWConfig config, IPUAlgoConfig algoconfig, int, size_t,
   scores.assign(total, 0);
   a_range_result.assign(total, 0);
   b_range_result.assign(total, 0);
+  // slot bookkeeping of the remote-buffer API (src/ipu_batch_affine.cpp:373-381)
+  slot_size_ = slotCap > 0 ? (int)slotCap : 1;
+  const int lots = algoconfig.transmissionPrograms > 0 ? algoconfig.transmissionPrograms : 1;
+  slots_.assign(lots, std::vector<bool>(slot_size_, false));
+  slot_avail_.assign(lots, slot_size_);
 }
 
 SWAlgorithm::SWAlgorithm(SWAlgorithm &&o) noexcept
     : algoconfig(o.algoconfig), config_(o.config_), engine_(o.engine_), scores(std::move(o.scores)),
-      a_range_result(std::move(o.a_range_result)), b_range_result(std::move(o.b_range_result)) {
+      a_range_result(std::move(o.a_range_result)), b_range_result(std::move(o.b_range_result)),
+      slot_size_(o.slot_size_), slot_avail_(std::move(o.slot_avail_)), slots_(std::move(o.slots_)) {
   o.engine_ = nullptr;
 }
 

@@ -147,6 +147,9 @@ hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_encode(const uint8_t *chars, int64_t len, uint8_t *codes, const uint8_t *table,
                          hipStream_t st);
 hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st);
+hipError_t launch_wire_gather(int64_t nseq, const uint8_t *wire, const int64_t *src,
+                              const int64_t *dst, const int32_t *len, const uint8_t *remap,
+                              uint8_t *arena, hipStream_t st);
 hipError_t launch_fast(int kp, const FastArgs &A, int blocks, hipStream_t st);
 int fast_blocks_per_cu(int kp);
 hipError_t launch_generic(const GenericArgs &G, int blocks, hipStream_t st);

@@ -12,10 +12,15 @@
 
 #include <algorithm>
 #include <climits>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/ipuma_hip.h"
@@ -86,6 +91,17 @@ struct ChainBufs {
   }
 };
 
+// A prepared batch queued for the engine's executor thread.
+struct PreparedJob {
+  const int32_t *in = nullptr;
+  int64_t in_words = 0;
+  int32_t *res = nullptr;
+  int64_t res_words = 0;
+  int rc = 0;
+  std::string err;
+  bool done = false;
+};
+
 }  // namespace
 
 struct ipuma_engine {
@@ -124,6 +140,23 @@ struct ipuma_engine {
   int timing = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // batch begin, fast begin/end, batch end
   bool ev_recorded = false;
+  // Host calls are serialized; device work of calls on different streams is
+  // ordered through order_ev (the engine's buffers are shared).
+  std::mutex mu;
+  hipEvent_t order_ev = nullptr;
+  // prepared input buffers (the reference's wire format)
+  uint8_t *d_wiretab = nullptr;  // code -> code inside the matrix
+  DevBuf w_wire, w_desc, w_arena, w_res;
+  std::vector<int64_t> h_src, h_dst;
+  std::vector<int32_t> h_len;
+  // executor thread of asynchronously submitted prepared batches
+  std::mutex pj_mu;
+  std::condition_variable pj_cv;
+  std::deque<int64_t> pj_queue;
+  std::unordered_map<int64_t, PreparedJob> pj_jobs;
+  int64_t pj_next = 1;
+  std::thread pj_thread;
+  bool pj_stop = false;
 };
 
 namespace {
@@ -152,6 +185,11 @@ int init_engine(ipuma_engine *e) {
                     hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&e->d_codetab, 256));
   HIP_TRY(hipMemcpy(e->d_codetab, e->h_codes, 256, hipMemcpyHostToDevice));
+  uint8_t wiretab[256];
+  for (int i = 0; i < 256; ++i) wiretab[i] = (uint8_t)(i < dim ? i : dim - 1);
+  HIP_TRY(hipMalloc(&e->d_wiretab, 256));
+  HIP_TRY(hipMemcpy(e->d_wiretab, wiretab, 256, hipMemcpyHostToDevice));
+  HIP_TRY(hipEventCreateWithFlags(&e->order_ev, hipEventDisableTiming));
   const int o = -(e->sw.gap_init + e->sw.gap_extend), x = -e->sw.gap_extend;
   e->packed_ok = o >= 0 && x >= 0 && o <= 32767 && x <= 32767;
   e->forward_only = e->algo.forward_only ? 1 : 0;
@@ -238,10 +276,42 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   return IPUMA_OK;
 }
 
+// Device work on `st` after the engine's previous work on any stream (its
+// buffers are shared); skipped while `st` is being captured into a graph.
+bool capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+int begin_work(ipuma_engine *e, hipStream_t st) {
+  if (!capturing(st)) HIP_TRY(hipStreamWaitEvent(st, e->order_ev, 0));
+  return IPUMA_OK;
+}
+int end_work(ipuma_engine *e, hipStream_t st) {
+  if (!capturing(st)) HIP_TRY(hipEventRecord(e->order_ev, st));
+  return IPUMA_OK;
+}
+
+int compare_device_chain(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
+                         const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
+                         const int32_t *b_len, int32_t *scores, int32_t *a_range,
+                         int32_t *b_range, hipStream_t st);
+
 int compare_device_impl(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
                         const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
                         const int32_t *b_len, int32_t *scores, int32_t *a_range,
                         int32_t *b_range, hipStream_t st) {
+  int rc = begin_work(e, st);
+  if (rc) return rc;
+  rc = compare_device_chain(e, n, codes, codes_len, a_off, a_len, b_off, b_len, scores, a_range,
+                            b_range, st);
+  if (rc) return rc;
+  return end_work(e, st);
+}
+
+int compare_device_chain(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
+                         const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
+                         const int32_t *b_len, int32_t *scores, int32_t *a_range,
+                         int32_t *b_range, hipStream_t st) {
   if (e->timing) HIP_TRY(hipEventRecord(e->ev[0], st));
   e->ev_recorded = false;
   int rc = run_chain(e, e->fwd, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st);
@@ -358,6 +428,117 @@ int check_sizes(ipuma_engine *e, int32_t n, const int64_t *a_lens_or_null,
   return IPUMA_OK;
 }
 
+// A prepared input buffer (the reference's wire format, laid out for the
+// engine's algo config and layout lot): the metadata is checked on the host,
+// the buffer is copied to HBM as is, the residues of every slot are gathered
+// to 4-aligned places, and the slots run as one device batch (empty slots
+// score 0).  The results buffer receives the first result_words words of
+// [scores | a ranges | b ranges], one word per slot.
+int prepared_impl(ipuma_engine *e, const int32_t *in, int64_t in_words, int32_t *res,
+                  int64_t res_words, int lot) {
+  const ipuma_algo_config &c = e->algo;
+  const int lots = c.transmission_programs;
+  if (c.tiles_used < 0 || c.max_batches < 0 || c.bufsize < 0 || lots < 1)
+    return fail(IPUMA_ECONFIG, "tiles_used, max_batches, bufsize >= 0 and transmission_programs >= 1 required");
+  if (lot < 0 || lot >= lots) return fail(IPUMA_EINVAL, "layout lot %d outside 0..%d", lot, lots - 1);
+  if (in_words < 0 || res_words < 0 || (in_words > 0 && !in) || (res_words > 0 && !res))
+    return fail(IPUMA_EINVAL, "bad buffers");
+  auto scale = [&](int64_t x) { return (x / lots + (x % lots == 0 ? 0 : 1)) * (int64_t)(lot + 1); };
+  const int64_t ibuf = ((int64_t)c.bufsize + 3) / 4;
+  const int64_t tile_bytes = scale(4 * ibuf);
+  const int64_t meta_off = scale(ibuf) * c.tiles_used;
+  const int64_t slots = (int64_t)c.tiles_used * c.max_batches;
+  if (slots > INT32_MAX - kSortedPad) return fail(IPUMA_EINVAL, "too many slots");
+  if (in_words < meta_off + 4 * slots)
+    return fail(IPUMA_EINVAL, "input buffer of %lld words is shorter than the layout's %lld",
+                (long long)in_words, (long long)(meta_off + 4 * slots));
+  if (res_words > 3 * slots)
+    return fail(IPUMA_EINVAL, "results buffer of %lld words exceeds 3 x %lld slots",
+                (long long)res_words, (long long)slots);
+  const int32_t *meta = in + meta_off;
+  e->h_src.resize(2 * slots);
+  e->h_dst.resize(2 * slots);
+  e->h_len.resize(2 * slots);
+  int64_t pos = 0;
+  for (int64_t q = 0; q < slots; ++q) {
+    const int64_t tile = q / std::max(c.max_batches, 1);
+    const int32_t la = meta[4 * q], oa = meta[4 * q + 1], lb = meta[4 * q + 2], ob = meta[4 * q + 3];
+    const bool empty = la == 0 || lb == 0;
+    if (!empty) {
+      if (la < 0 || lb < 0) return fail(IPUMA_EINVAL, "slot %lld: negative length", (long long)q);
+      if (la > kMaxSeqLen || lb > kMaxSeqLen)
+        return fail(IPUMA_ETOOLONG, "slot %lld: a sequence longer than %d", (long long)q, kMaxSeqLen);
+      if (oa < 0 || ob < 0 || (int64_t)oa + la > tile_bytes || (int64_t)ob + lb > tile_bytes)
+        return fail(IPUMA_EINVAL, "slot %lld: a sequence outside its tile's %lld bytes",
+                    (long long)q, (long long)tile_bytes);
+    }
+    const int32_t na = empty ? 0 : la, nb = empty ? 0 : lb;
+    e->h_len[q] = na;
+    e->h_len[slots + q] = nb;
+    e->h_src[q] = tile * tile_bytes + (empty ? 0 : oa);
+    e->h_src[slots + q] = tile * tile_bytes + (empty ? 0 : ob);
+    e->h_dst[q] = pos;
+    pos += (na + 3) / 4 * 4;
+    e->h_dst[slots + q] = pos;
+    pos += (nb + 3) / 4 * 4;
+  }
+  const int64_t arena = pos + 16;
+  const size_t n2 = (size_t)(2 * slots);
+  int rc;
+  if ((rc = e->w_wire.ensure((size_t)in_words * 4)) || (rc = e->w_desc.ensure(n2 * 20)) ||
+      (rc = e->w_arena.ensure((size_t)arena)) || (rc = e->w_res.ensure((size_t)slots * 12 + 16)))
+    return rc;
+  int64_t *d_src = e->w_desc.as<int64_t>(), *d_dst = d_src + n2;
+  int32_t *d_len = reinterpret_cast<int32_t *>(d_dst + n2);
+  int32_t *d_res = e->w_res.as<int32_t>();
+  hipStream_t st = e->stream;
+  if ((rc = begin_work(e, st))) return rc;
+  if (slots > 0) {
+    HIP_TRY(hipMemcpyAsync(e->w_wire.p, in, (size_t)in_words * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_src, e->h_src.data(), n2 * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_dst, e->h_dst.data(), n2 * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_len, e->h_len.data(), n2 * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_wire_gather((int64_t)n2, e->w_wire.as<uint8_t>(), d_src, d_dst, d_len,
+                               e->d_wiretab, e->w_arena.as<uint8_t>(), st));
+  }
+  rc = compare_device_chain(e, (int32_t)slots, e->w_arena.as<uint8_t>(), arena, d_dst, d_len,
+                            d_dst + slots, d_len + slots, d_res, d_res + slots, d_res + 2 * slots,
+                            st);
+  if (rc) return rc;
+  if ((rc = end_work(e, st))) return rc;
+  if (res_words > 0)
+    HIP_TRY(hipMemcpyAsync(res, d_res, (size_t)res_words * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return check_device_error(e);
+}
+
+// Executor of asynchronously submitted prepared batches, in submission order.
+void executor_loop(ipuma_engine *e) {
+  for (;;) {
+    int64_t id;
+    PreparedJob job;
+    {
+      std::unique_lock<std::mutex> lk(e->pj_mu);
+      e->pj_cv.wait(lk, [&] { return e->pj_stop || !e->pj_queue.empty(); });
+      if (e->pj_queue.empty()) return;  // stopping, nothing left
+      id = e->pj_queue.front();
+      e->pj_queue.pop_front();
+      job = e->pj_jobs[id];
+    }
+    int rc;
+    {
+      std::lock_guard<std::mutex> g(e->mu);
+      rc = prepared_impl(e, job.in, job.in_words, job.res, job.res_words, 0);
+    }
+    std::lock_guard<std::mutex> lk(e->pj_mu);
+    PreparedJob &j = e->pj_jobs[id];
+    j.rc = rc;
+    if (rc) j.err = g_last_error;
+    j.done = true;
+    e->pj_cv.notify_all();
+  }
+}
+
 }  // namespace
 
 namespace ipuma {
@@ -427,6 +608,14 @@ int ipuma_engine_create(const ipuma_sw_config *sw, const ipuma_algo_config *algo
 
 void ipuma_engine_destroy(ipuma_engine *e) {
   if (!e) return;
+  if (e->pj_thread.joinable()) {  // queued prepared batches still run
+    {
+      std::lock_guard<std::mutex> lk(e->pj_mu);
+      e->pj_stop = true;
+    }
+    e->pj_cv.notify_all();
+    e->pj_thread.join();
+  }
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   e->scratch.release();
   e->band_scratch.release();
@@ -442,6 +631,12 @@ void ipuma_engine_destroy(ipuma_engine *e) {
   e->in_codes.release();
   e->in_meta.release();
   e->out_res.release();
+  e->w_wire.release();
+  e->w_desc.release();
+  e->w_arena.release();
+  e->w_res.release();
+  if (e->d_wiretab) (void)hipFree(e->d_wiretab);
+  if (e->order_ev) (void)hipEventDestroy(e->order_ev);
   for (auto &ev : e->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (e->d_table) (void)hipFree(e->d_table);
@@ -454,6 +649,7 @@ int ipuma_compare_local(ipuma_engine *e, int32_t n, const char *a_chars, const i
                         const char *b_chars, const int64_t *b_offsets, int32_t *scores,
                         int32_t *a_range, int32_t *b_range) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
   if (n < 0) return fail(IPUMA_EINVAL, "n < 0");
   if (n > 0 && (!a_offsets || !b_offsets || !scores || !a_range || !b_range))
     return fail(IPUMA_EINVAL, "NULL buffer");
@@ -489,6 +685,7 @@ int ipuma_compare_mn_local(ipuma_engine *e, int32_t nseqs, const char *chars,
                            const int64_t *offsets, int32_t ncmps, const int32_t *cmps,
                            int32_t *scores, int32_t *a_range, int32_t *b_range) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
   if (nseqs < 0 || ncmps < 0) return fail(IPUMA_EINVAL, "negative count");
   if (ncmps > 0 && (!cmps || !scores || !a_range || !b_range || !offsets))
     return fail(IPUMA_EINVAL, "NULL buffer");
@@ -528,6 +725,7 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
                          const int32_t *b_len, int32_t *scores, int32_t *a_range,
                          int32_t *b_range, void *stream) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
   if (n < 0) return fail(IPUMA_EINVAL, "n < 0");
   if (n > 0 && (!codes || !a_off || !a_len || !b_off || !b_len || !scores || !a_range || !b_range))
     return fail(IPUMA_EINVAL, "NULL buffer");
@@ -541,6 +739,7 @@ int ipuma_search_device(ipuma_engine *e, const uint8_t *codes, int64_t codes_len
                         const int64_t *db_off, const int32_t *db_len, int32_t k,
                         int32_t *scores, int32_t *top_idx, int32_t *top_score, void *stream) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
   if (nq < 0 || ndb < 0 || k < 0 || k > 1024)
     return fail(IPUMA_EINVAL, "nq, ndb >= 0 and 0 <= k <= 1024 required");
   const int64_t n = (int64_t)nq * ndb;
@@ -557,6 +756,7 @@ int ipuma_search_device(ipuma_engine *e, const uint8_t *codes, int64_t codes_len
   int32_t *a_len = reinterpret_cast<int32_t *>(b_off + n), *b_len = a_len + n;
   int32_t *res = e->s_res.as<int32_t>();
   int32_t *sc = scores ? scores : res + 2 * n;
+  if ((rc = begin_work(e, st))) return rc;
   HIP_TRY(launch_search_pairs(nq, ndb, q_off, q_len, db_off, db_len, a_off, a_len, b_off, b_len,
                               st));
   // scores only: the forward pass (its end positions are discarded)
@@ -567,13 +767,14 @@ int ipuma_search_device(ipuma_engine *e, const uint8_t *codes, int64_t codes_len
   e->forward_only = saved;
   if (rc) return rc;
   HIP_TRY(launch_topk(nq, ndb, k, sc, top_idx, top_score, st));
-  return IPUMA_OK;
+  return end_work(e, st);
 }
 
 int ipuma_set_batching(ipuma_engine *e, int32_t mode) {
   if (!e || (mode != IPUMA_BATCH_BUCKETED && mode != IPUMA_BATCH_PLAIN &&
              mode != IPUMA_BATCH_PLAIN_TILE))
     return fail(IPUMA_EINVAL, "invalid batching mode");
+  std::lock_guard<std::mutex> lock_(e->mu);
   e->batching = mode;
   return IPUMA_OK;
 }
@@ -581,18 +782,21 @@ int ipuma_set_batching(ipuma_engine *e, int32_t mode) {
 int ipuma_set_precision(ipuma_engine *e, int32_t mode) {
   if (!e || (mode != IPUMA_PREC_AUTO && mode != IPUMA_PREC_INT32))
     return fail(IPUMA_EINVAL, "invalid precision mode");
+  std::lock_guard<std::mutex> lock_(e->mu);
   e->precision = mode;
   return IPUMA_OK;
 }
 
 int ipuma_set_forward_only(ipuma_engine *e, int32_t forward_only) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
   e->forward_only = forward_only ? 1 : 0;
   return IPUMA_OK;
 }
 
 int ipuma_set_timing(ipuma_engine *e, int32_t on) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
   if (on && !e->ev[0])
     for (auto &ev : e->ev) HIP_TRY(hipEventCreate(&ev));
   e->timing = on ? 1 : 0;
@@ -601,6 +805,7 @@ int ipuma_set_timing(ipuma_engine *e, int32_t on) {
 
 int ipuma_set_work_rounds(ipuma_engine *e, int32_t rounds) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
   if (rounds < 1 || rounds > 64) return fail(IPUMA_EINVAL, "rounds must be in 1..64");
   e->rounds = rounds;
   e->target_slots = (int64_t)e->fast_blocks * (kFastThreads / 64) * 128 * e->rounds;
@@ -609,6 +814,7 @@ int ipuma_set_work_rounds(ipuma_engine *e, int32_t rounds) {
 
 int ipuma_set_lanes_per_pair(ipuma_engine *e, int32_t lanes) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
   if (lanes == 0) {
     e->force_class = -1;
     return IPUMA_OK;
@@ -624,6 +830,7 @@ int ipuma_set_lanes_per_pair(ipuma_engine *e, int32_t lanes) {
 int ipuma_encode_device(ipuma_engine *e, const char *d_chars, int64_t len, uint8_t *d_codes,
                         void *stream) {
   if (!e || len < 0 || (len > 0 && (!d_chars || !d_codes))) return fail(IPUMA_EINVAL, "bad args");
+  std::lock_guard<std::mutex> lock_(e->mu);
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : e->stream;
   HIP_TRY(launch_encode(reinterpret_cast<const uint8_t *>(d_chars), len, d_codes, e->d_codetab, st));
   return IPUMA_OK;
@@ -631,6 +838,7 @@ int ipuma_encode_device(ipuma_engine *e, const char *d_chars, int64_t len, uint8
 
 int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out) {
   if (!e || !out) return fail(IPUMA_EINVAL, "bad args");
+  std::lock_guard<std::mutex> lock_(e->mu);
   std::memset(out, 0, sizeof(*out));
   if (!e->last_ctl) return IPUMA_OK;
   HIP_TRY(hipStreamSynchronize(e->stream));
@@ -658,8 +866,54 @@ int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out) {
   return IPUMA_OK;
 }
 
+int ipuma_prepared_compare(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
+                           int32_t *results, int64_t result_words, int32_t lot) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
+  return prepared_impl(e, inputs, input_words, results, result_words, lot);
+}
+
+int ipuma_prepared_submit(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
+                          int32_t *results, int64_t result_words, int64_t *job) {
+  if (!e || !job) return fail(IPUMA_EINVAL, "engine or job is NULL");
+  if (input_words < 0 || result_words < 0 || (input_words > 0 && !inputs) ||
+      (result_words > 0 && !results))
+    return fail(IPUMA_EINVAL, "bad buffers");
+  std::lock_guard<std::mutex> lk(e->pj_mu);
+  if (e->pj_stop) return fail(IPUMA_ECLOSED, "engine is closing");
+  if (!e->pj_thread.joinable()) e->pj_thread = std::thread(executor_loop, e);
+  const int64_t id = e->pj_next++;
+  PreparedJob &j = e->pj_jobs[id];
+  j.in = inputs;
+  j.in_words = input_words;
+  j.res = results;
+  j.res_words = result_words;
+  e->pj_queue.push_back(id);
+  e->pj_cv.notify_all();
+  *job = id;
+  return IPUMA_OK;
+}
+
+int ipuma_prepared_join(ipuma_engine *e, int64_t job) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::unique_lock<std::mutex> lk(e->pj_mu);
+  auto it = e->pj_jobs.find(job);
+  if (it == e->pj_jobs.end()) return fail(IPUMA_EINVAL, "unknown or already joined job %lld", (long long)job);
+  e->pj_cv.wait(lk, [&] {
+    auto f = e->pj_jobs.find(job);
+    return f == e->pj_jobs.end() || f->second.done;
+  });
+  it = e->pj_jobs.find(job);
+  if (it == e->pj_jobs.end()) return fail(IPUMA_EINVAL, "job %lld joined twice", (long long)job);
+  const PreparedJob j = it->second;
+  e->pj_jobs.erase(it);
+  if (j.rc) return fail(j.rc, "%s", j.err.c_str());
+  return IPUMA_OK;
+}
+
 int ipuma_synchronize(ipuma_engine *e) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipDeviceSynchronize());
   return check_device_error(e);

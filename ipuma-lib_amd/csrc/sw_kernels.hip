@@ -1330,6 +1330,27 @@ __global__ __launch_bounds__(256) void k_plain_groups(int32_t n, int32_t per_gro
 }
 
 // ---------------------------------------------------------------------------
+// Prepared input buffers (the reference's wire format): the residues of each
+// slot's A and B are copied from their tile regions to 4-aligned places of a
+// codes arena, codes outside the scoring matrix mapped by `remap` (as the
+// character path maps them).  Sequence q (A of slot q, or B of slot q - slots)
+// is copied by one wave.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_wire_gather(int64_t nseq, const uint8_t *wire,
+                                                     const int64_t *src, const int64_t *dst,
+                                                     const int32_t *len, const uint8_t *remap,
+                                                     uint8_t *arena) {
+  const int lane = threadIdx.x & 63;
+  const int64_t step = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nseq; q += step) {
+    const int l = len[q];
+    const uint8_t *s = wire + src[q];
+    uint8_t *d = arena + dst[q];
+    for (int i = lane; i < l; i += 64) d[i] = remap[s[i]];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Launch wrappers (host side).
 // ---------------------------------------------------------------------------
 static inline int nblocks(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -1371,6 +1392,16 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
   hipLaunchKernelGGL(k_scan_cost, dim3(1), dim3(1024), 0, st, B.costhist);
   hipLaunchKernelGGL(k_order_groups, dim3(std::min(nblocks(B.max_groups, 256), 512)), dim3(256), 0,
                      st, B.max_groups, B.ctl, B.gtmp, B.gbin, B.grank, B.costhist, B.groups);
+  return hipGetLastError();
+}
+
+hipError_t launch_wire_gather(int64_t nseq, const uint8_t *wire, const int64_t *src,
+                              const int64_t *dst, const int32_t *len, const uint8_t *remap,
+                              uint8_t *arena, hipStream_t st) {
+  if (nseq <= 0) return hipSuccess;
+  const int blocks = (int)std::min<int64_t>(nblocks(nseq * 64, 256), 8192);
+  hipLaunchKernelGGL(k_wire_gather, dim3(blocks), dim3(256), 0, st, nseq, wire, src, dst, len,
+                     remap, arena);
   return hipGetLastError();
 }
 

@@ -164,6 +164,14 @@ def lib():
     L.ipuma_server_wait.argtypes = [_P, _i64, _P, _P, _P, ctypes.POINTER(ctypes.c_double)]
     L.ipuma_server_get_stats.argtypes = [_P, ctypes.POINTER(ServerStats)]
     L.ipuma_server_engine.argtypes = [_P, ctypes.POINTER(_P)]
+    L.ipuma_input_buffer_words.argtypes = [_P]
+    L.ipuma_input_buffer_words.restype = _i64
+    L.ipuma_prepare_remote.argtypes = [_P, _P, _i32, _P, _P, _P, _P, _P, _i64, _P, _P]
+    L.ipuma_prepare_remote_mn.argtypes = [_P, _P, _i32, _P, _P, _i32, _P, _P, _i64, _P, _P]
+    L.ipuma_transfer_results.argtypes = [_P, _i64, _P, _i32, _P, _P, _P]
+    L.ipuma_prepared_compare.argtypes = [_P, _P, _i64, _P, _i64, _i32]
+    L.ipuma_prepared_submit.argtypes = [_P, _P, _i64, _P, _i64, ctypes.POINTER(_i64)]
+    L.ipuma_prepared_join.argtypes = [_P, _i64]
     return L
 
 
@@ -176,8 +184,8 @@ def header_symbols(path: str = HEADER):
     """Function names the C ABI header declares."""
     import re
     text = open(path).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(ipuma_\w+)\s*\(",
-                                 text, re.M)))
+    return sorted(set(re.findall(
+        r"^\s*(?:int64_t|int|void|const char \*)\s*\*?\s*(ipuma_\w+)\s*\(", text, re.M)))
 
 
 def _pack(seqs):
@@ -187,6 +195,81 @@ def _pack(seqs):
         offs[1:] = np.cumsum([len(x) for x in enc])
     buf = b"".join(enc) + b"\0"
     return buf, offs
+
+
+# --- prepared input buffers (the reference's wire format) --------------------
+def input_buffer_words(algo: IPUAlgoConfig) -> int:
+    """IPUAlgoConfig::getInputBufferSize32b (src/ipu_config.cpp:14)."""
+    a = algo._c()
+    return int(lib().ipuma_input_buffer_words(ctypes.byref(a)))
+
+
+def _i32buf(x, n=None):
+    if x is None:
+        return np.zeros(n, np.int32)
+    if not (isinstance(x, np.ndarray) and x.dtype == np.int32 and x.flags.c_contiguous):
+        raise TypeError("expected a contiguous numpy int32 array")
+    return x
+
+
+def prepare_remote(sw: SWConfig, algo: IPUAlgoConfig, A, B, inputs=None):
+    """SWAlgorithm::prepare_remote (src/ipu_batch_affine.cpp:891-950): the input
+    buffer (numpy int32, input_buffer_words long unless given), the slot of
+    every comparison and the largest tile fill."""
+    if len(A) != len(B):
+        raise ValueError("A and B must have the same number of sequences")
+    inputs = _i32buf(inputs, input_buffer_words(algo))
+    mapping = np.zeros(len(A), np.int32)
+    abuf, aoff = _pack(A)
+    bbuf, boff = _pack(B)
+    mb = ctypes.c_int32(0)
+    s, a = sw._c(), algo._c()
+    _check(lib().ipuma_prepare_remote(ctypes.byref(s), ctypes.byref(a), len(A), abuf,
+                                      aoff.ctypes.data, bbuf, boff.ctypes.data,
+                                      inputs.ctypes.data, len(inputs), mapping.ctypes.data,
+                                      ctypes.byref(mb)))
+    return inputs, mapping, mb.value
+
+
+def prepare_remote_mn(sw: SWConfig, algo: IPUAlgoConfig, seqs, cmps, inputs=None):
+    """compare_mn_local's preparation (fillMNBuckets + fill_input_buffer)."""
+    inputs = _i32buf(inputs, input_buffer_words(algo))
+    c = np.ascontiguousarray(np.asarray(cmps, np.int32).reshape(-1, 2))
+    mapping = np.zeros(len(c), np.int32)
+    buf, offs = _pack(seqs)
+    mb = ctypes.c_int32(0)
+    s, a = sw._c(), algo._c()
+    _check(lib().ipuma_prepare_remote_mn(ctypes.byref(s), ctypes.byref(a), len(seqs), buf,
+                                         offs.ctypes.data, len(c), c.ctypes.data,
+                                         inputs.ctypes.data, len(inputs), mapping.ctypes.data,
+                                         ctypes.byref(mb)))
+    return inputs, mapping, mb.value
+
+
+def transfer_results(results, mapping):
+    """SWAlgorithm::transferResults (src/ipu_batch_affine.cpp:674-695)."""
+    results = _i32buf(results)
+    mapping = np.ascontiguousarray(mapping, np.int32)
+    n = len(mapping)
+    out = [np.zeros(n, np.int32) for _ in range(3)]
+    _check(lib().ipuma_transfer_results(results.ctypes.data, len(results), mapping.ctypes.data,
+                                        n, *[o.ctypes.data for o in out]))
+    return tuple(out)
+
+
+class PreparedJob:
+    """A submitted prepared buffer (src/ipu_batch_affine.h:41-48); holds its
+    buffers until joined."""
+
+    def __init__(self, engine, job_id, inputs, results):
+        self._engine, self.id, self.inputs, self.results = engine, job_id, inputs, results
+        self.joined = False
+
+    def join(self):
+        if not self.joined:
+            self.joined = True
+            _check(lib().ipuma_prepared_join(self._engine._h, self.id))
+        return self.results
 
 
 @dataclass
@@ -208,6 +291,7 @@ class SWAlgorithm:
         al = self.algoconfig._c()
         _check(lib().ipuma_engine_create(ctypes.byref(sw), ctypes.byref(al), ctypes.byref(h)))
         self._h = h
+        self.slot_cap = 1  # remote-buffer slots per layout lot (slotToken = lot * slot_cap + slot)
         n = self.algoconfig.tilesUsed * self.algoconfig.maxBatches
         self._result = BlockAlignmentResults(np.zeros(n, np.int32), np.zeros(n, np.int32),
                                              np.zeros(n, np.int32))
@@ -271,6 +355,33 @@ class SWAlgorithm:
         """compare_mn_local (src/ipu_batch_affine.cpp:697-750)."""
         s, a, b = self.compare_mn(seqs, cmps)
         self._store(s, a, b)
+
+    def prepared_remote_compare(self, inputs, results=None, slot_token: int = 0):
+        """prepared_remote_compare (src/ipu_batch_affine.cpp:557-562): runs a
+        prepared buffer; returns the results buffer (3 x tilesUsed*maxBatches
+        words unless given: [scores | a ranges | b ranges])."""
+        inputs = _i32buf(inputs)
+        n = self.algoconfig.tilesUsed * self.algoconfig.maxBatches
+        results = _i32buf(results, 3 * n)
+        lot = int(slot_token) // self.slot_cap
+        _check(lib().ipuma_prepared_compare(self._h, inputs.ctypes.data, len(inputs),
+                                            results.ctypes.data, len(results), lot))
+        return results
+
+    def async_submit_prepared_remote_compare(self, inputs, results=None) -> PreparedJob:
+        """async_submit_prepared_remote_compare (src/ipu_batch_affine.cpp:496-505)."""
+        inputs = _i32buf(inputs)
+        n = self.algoconfig.tilesUsed * self.algoconfig.maxBatches
+        results = _i32buf(results, 3 * n)
+        job = ctypes.c_int64(0)
+        _check(lib().ipuma_prepared_submit(self._h, inputs.ctypes.data, len(inputs),
+                                           results.ctypes.data, len(results),
+                                           ctypes.byref(job)))
+        return PreparedJob(self, job.value, inputs, results)
+
+    def blocking_join_prepared_remote_compare(self, job: PreparedJob):
+        """blocking_join_prepared_remote_compare (src/ipu_batch_affine.cpp:507-555)."""
+        return job.join()
 
     def get_result(self) -> BlockAlignmentResults:
         """get_result (src/ipu_batch_affine.cpp:408): copies sized tilesUsed*maxBatches."""

@@ -2,6 +2,7 @@
 // as a plain C++ program over ipu::batchaffine::SWAlgorithm (the facade of the
 // MI355X engine).  Rows and expectations: test/simple-test.cpp:25-45, 105-171;
 // test/mn_cmp-test.cpp:29-57.  Exit code 0 when every check passes.
+#include <algorithm>
 #include <cstdio>
 #include <stdexcept>
 #include <cstdint>
@@ -137,6 +138,48 @@ int main() {
       threw = true;
     }
     EXPECT_EQ(threw, true, "|A| > maxAB throws");
+  }
+  {  // prepared buffers: prepare_remote, prepared_remote_compare, transferResults (the
+     // flow of the reference's batch driver, ipusw/batch.hpp:34-63, ipusw/ipuworker.hpp:8-66),
+     // then the same buffers through the asynchronous job API
+    ipu::IPUAlgoConfig algo;
+    algo.tilesUsed = 3;
+    algo.maxAB = 300;
+    algo.maxBatches = 10;
+    algo.bufsize = 1000;
+    for (auto fill : {ipu::Algorithm::fillFirst, ipu::Algorithm::roundRobin, ipu::Algorithm::greedy}) {
+      algo.fillAlgo = fill;
+      ipu::batchaffine::SWAlgorithm driver({}, algo);
+      std::vector<int32_t> inputs(algo.getInputBufferSize32b());
+      std::vector<int> mapping(queries.size());
+      ipu::batchaffine::SWAlgorithm::prepare_remote({}, algo, queries, refs, inputs.data(),
+                                                    inputs.data() + inputs.size(), mapping.data());
+      std::vector<int32_t> results(3 * algo.getTotalNumberOfComparisons());
+      ipu::batchaffine::BlockAlignmentResults r;
+      const size_t n = queries.size();
+      r.scores.resize(n);
+      r.a_range_result.resize(n);
+      r.b_range_result.resize(n);
+      for (int pass = 0; pass < 2; ++pass) {
+        std::fill(results.begin(), results.end(), -7);
+        if (pass == 0) {
+          driver.prepared_remote_compare(inputs.data(), inputs.data() + inputs.size(),
+                                         results.data(), results.data() + results.size());
+        } else {
+          auto *job = driver.async_submit_prepared_remote_compare(
+              inputs.data(), inputs.data() + inputs.size(), results.data(),
+              results.data() + results.size());
+          driver.blocking_join_prepared_remote_compare(*job);
+          delete job;
+        }
+        ipu::batchaffine::SWAlgorithm::transferResults(
+            results.data(), results.data() + results.size(), mapping.data(),
+            mapping.data() + mapping.size(), r.scores.data(), r.scores.data() + n,
+            r.a_range_result.data(), r.a_range_result.data() + n, r.b_range_result.data(),
+            r.b_range_result.data() + n);
+        check_rows(r, pass == 0 ? "prepared" : "prepared async");
+      }
+    }
   }
   {  // driver.hpp singleton with the compile-time configuration
     init_single_ipu(SW_CONFIGURATION, ALGO_CONFIGURATION);

@@ -63,6 +63,11 @@ def emu_lib(build_dir):
         o = os.path.join(build_dir, src + ".o")
         _compile(os.path.join(CSRC, src), o, extra)
         objs.append(o)
+    cpp = os.path.join(ROOT, "ipuma-lib_amd", "cpp")
+    for src in sorted(f for f in os.listdir(cpp) if f.endswith(".cpp")):
+        o = os.path.join(build_dir, "cpp_" + src + ".o")
+        _compile(os.path.join(cpp, src), o, ["-I" + cpp])
+        objs.append(o)
     lib_path = os.path.join(build_dir, "libipuma_emu.so")
     subprocess.run([CLANG, "-shared", "-pthread", *objs, "-o", lib_path], check=True)
     lib = ctypes.CDLL(lib_path)
@@ -218,4 +223,69 @@ def test_band_groups_emulated(emu_lib):
     exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1, forward_only=False)
     for got, want in zip((s, ar, br), exp):
         assert np.array_equal(np.asarray(got), np.asarray(want))
+    eng.close()
+
+
+def test_prepared_buffers_emulated(emu_lib):
+    """The wire-format path through the emulated engine: prepare_remote (fill
+    first and greedy), prepared_remote_compare, transferResults, against the
+    oracle; a comparison-list buffer; and three buffers submitted
+    asynchronously from two threads, joined in another order."""
+    import threading
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(31)
+    sw = I.SWConfig(gapInit=-2, gapExtend=-1)
+    A = ["".join(rng.choice(list("ACGTN"), int(rng.integers(1, 70)))) for _ in range(30)]
+    B = ["".join(rng.choice(list("ACGT"), int(rng.integers(1, 90)))) + A[k][: len(A[k]) // 2]
+         for k in range(30)]
+    exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1)
+    for fill in ("fillfirst", "greedy"):
+        algo = I.IPUAlgoConfig(tilesUsed=8, maxAB=200, maxBatches=5, bufsize=700, fillAlgo=fill)
+        eng = I.SWAlgorithm(sw, algo)
+        inputs, mapping, _ = I.prepare_remote(sw, algo, A, B)
+        res = eng.prepared_remote_compare(inputs)
+        got = I.transfer_results(res, mapping)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e), fill
+        # unused slots score 0
+        used = np.zeros(8 * 5, bool)
+        used[mapping] = True
+        assert not res[:40][~used].any()
+        eng.close()
+    seqs = A[:10] + B[:10]
+    cmps = [(i, 10 + (i * 3) % 10) for i in range(10)] + [(2, 2), (5, 12), (5, 13)]
+    algo = I.IPUAlgoConfig(tilesUsed=3, maxAB=200, maxBatches=6, bufsize=800)
+    eng = I.SWAlgorithm(sw, algo)
+    inputs, mapping, _ = I.prepare_remote_mn(sw, algo, seqs, cmps)
+    got = I.transfer_results(eng.prepared_remote_compare(inputs), mapping)
+    expm = O.sw_batch([seqs[a] for a, _ in cmps], [seqs[b] for _, b in cmps], gap_init=-2,
+                      gap_extend=-1)
+    for g, e in zip(got, expm):
+        assert np.array_equal(g, e)
+    eng.close()
+    algo = I.IPUAlgoConfig(tilesUsed=4, maxAB=200, maxBatches=5, bufsize=700, forwardOnly=True)
+    eng = I.SWAlgorithm(sw, algo)
+    parts = [(A[i::3], B[i::3]) for i in range(3)]
+    prepared = [I.prepare_remote(sw, algo, a, b) for a, b in parts]
+    jobs = [None] * 3
+
+    def submit(idx):
+        for i in idx:
+            jobs[i] = eng.async_submit_prepared_remote_compare(prepared[i][0])
+
+    th = [threading.Thread(target=submit, args=(ix,)) for ix in ([0, 2], [1])]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i in (2, 0, 1):
+        res = eng.blocking_join_prepared_remote_compare(jobs[i])
+        s, ar, br = I.transfer_results(res, prepared[i][1])
+        es, ea, eb = O.sw_batch(parts[i][0], parts[i][1], gap_init=-2, gap_extend=-1,
+                                forward_only=True)
+        assert np.array_equal(s, es) and np.array_equal(ar, ea) and np.array_equal(br, eb)
+    with pytest.raises(I.IpumaError):
+        jobs[0].joined = False
+        jobs[0].join()  # joined twice
     eng.close()

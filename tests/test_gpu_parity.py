@@ -232,6 +232,74 @@ def test_band_groups(gpu):
         assert_same(got, exp, f"band groups, lanes={lanes}")
 
 
+def test_prepared_buffers(gpu):
+    # the reference's wire format end to end: prepare_remote with each fill
+    # algorithm, prepared_remote_compare, transferResults; a comparison-list
+    # buffer; asynchronous jobs from two threads
+    import threading
+    note("prepared buffers: 5000 DNA + 2000 protein pairs, three fills, MN, async")
+    rng = np.random.default_rng(23)
+
+    def strings(n, letters, lo, hi):
+        return ["".join(rng.choice(letters, int(rng.integers(lo, hi)))) for _ in range(n)]
+
+    dna = list("ACGT")
+    A = strings(5000, dna, 80, 200)
+    B = [strings(1, dna, 20, 80)[0] + a + strings(1, dna, 0, 60)[0] for a in A]
+    aa = list("ARNDCQEGHILKMFPSTWYV")
+    PA = strings(2000, aa, 50, 400)
+    PB = [a[: len(a) // 2] + strings(1, aa, 10, 200)[0] for a in PA]
+    cases = [(I.SWConfig(gapInit=-2, gapExtend=-1), A, B, O.SIM_NA, O.DT_NA),
+             (I.SWConfig(gapInit=-10, gapExtend=-1, similarity="blosum62", datatype="aa"), PA,
+              PB, O.SIM_BLOSUM62, O.DT_AA)]
+    for sw, a, b, sim, dt in cases:
+        exp = O.sw_batch(a, b, similarity=sim, datatype=dt, gap_init=sw.gapInit,
+                         gap_extend=sw.gapExtend)
+        for fill in ("fillfirst", "roundrobin", "greedy"):
+            algo = I.IPUAlgoConfig(tilesUsed=1472, maxAB=1000, maxBatches=4, bufsize=4000,
+                                   fillAlgo=fill)
+            eng = I.SWAlgorithm(sw, algo)
+            inputs, mapping, _ = I.prepare_remote(sw, algo, a, b)
+            got = I.transfer_results(eng.prepared_remote_compare(inputs), mapping)
+            assert_same(got, exp, f"prepared {sw.datatype} {fill}")
+            eng.close()
+    # comparison list: every A against two B's
+    seqs = A[:300] + B[:300]
+    cmps = [(i, 300 + i) for i in range(300)] + [(i, 300 + (i + 7) % 300) for i in range(300)]
+    sw = I.SWConfig(gapInit=-2, gapExtend=-1)
+    algo = I.IPUAlgoConfig(tilesUsed=200, maxAB=1000, maxBatches=6, bufsize=2500)
+    eng = I.SWAlgorithm(sw, algo)
+    inputs, mapping, _ = I.prepare_remote_mn(sw, algo, seqs, cmps)
+    got = I.transfer_results(eng.prepared_remote_compare(inputs), mapping)
+    assert_same(got, O.sw_batch([seqs[x] for x, _ in cmps], [seqs[y] for _, y in cmps],
+                                gap_init=-2, gap_extend=-1), "prepared mn")
+    # asynchronous jobs
+    algo = I.IPUAlgoConfig(tilesUsed=600, maxAB=1000, maxBatches=4, bufsize=4000,
+                           forwardOnly=True)
+    eng = I.SWAlgorithm(sw, algo)
+    parts = [(A[i::4], B[i::4]) for i in range(4)]
+    prepared = [I.prepare_remote(sw, algo, x, y) for x, y in parts]
+    jobs = [None] * 4
+
+    def submit(ix):
+        for i in ix:
+            jobs[i] = eng.async_submit_prepared_remote_compare(prepared[i][0])
+
+    th = [threading.Thread(target=submit, args=(ix,)) for ix in ([0, 2], [1, 3])]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i in (3, 1, 0, 2):
+        got = I.transfer_results(eng.blocking_join_prepared_remote_compare(jobs[i]),
+                                 prepared[i][1])
+        assert_same(got, O.sw_batch(parts[i][0], parts[i][1], gap_init=-2, gap_extend=-1,
+                                    forward_only=True), f"async job {i}")
+    with pytest.raises(I.IpumaError):
+        eng.prepared_remote_compare(prepared[0][0][:100])  # shorter than the layout
+    eng.close()
+
+
 def test_int16_saturation_rerun(gpu):
     # W^L vs W^L under BLOSUM62 scores 11*L: 45056 at L = 4096 overflows int16
     note("int16 saturation -> int32 re-run (W^4096)")
