@@ -197,9 +197,10 @@ def test_search_topk_emulated(emu_lib):
 
 def test_band_groups_emulated(emu_lib):
     """Whole-wave groups of two or more super-strips (|B| > 1024) shared by the
-    waves of a block: long DNA pairs among short ones, scores and both
-    ranges (the start pass runs the band groups again on the reversed
-    prefixes), against the oracle."""
+    waves of a block, at the classifier's pipeline depth and with every pair
+    on 64 lanes: long DNA pairs among short ones, scores and both ranges (the
+    start pass runs the band groups again on the reversed prefixes), against
+    the oracle."""
     import oracle as O
     I = emu_lib
     rng = np.random.default_rng(21)
@@ -215,15 +216,18 @@ def test_band_groups_emulated(emu_lib):
     for i in range(24):
         A.append("".join(rng.choice(list("ACGT"), int(rng.integers(5, 60)))))
         B.append("".join(rng.choice(list("ACGT"), int(rng.integers(5, 60)))))
-    eng = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
-                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
-    s, ar, br = eng.compare(A, B)
-    st = eng.stats()
-    assert st["band_groups"] >= 2, st
     exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1, forward_only=False)
-    for got, want in zip((s, ar, br), exp):
-        assert np.array_equal(np.asarray(got), np.asarray(want))
-    eng.close()
+    for lanes in (0, 64):  # the classifier's choice, then every pair on 64 lanes
+        eng = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
+                            I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+        if lanes:
+            eng.set_lanes_per_pair(lanes)
+        s, ar, br = eng.compare(A, B)
+        st = eng.stats()
+        assert st["band_groups"] >= 2, (lanes, st)
+        for got, want in zip((s, ar, br), exp):
+            assert np.array_equal(np.asarray(got), np.asarray(want)), lanes
+        eng.close()
 
 
 def test_prepared_buffers_emulated(emu_lib):
