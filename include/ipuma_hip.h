@@ -202,6 +202,24 @@ int ipuma_prepared_submit(ipuma_engine *e, const int32_t *inputs, int64_t input_
                           int32_t *results, int64_t result_words, int64_t *job);
 int ipuma_prepared_join(ipuma_engine *e, int64_t job);
 
+/*
+ * Traceback (no counterpart in the reference's IPU path; its CPU comparator,
+ * SSW, reports one: extern/include/ssw/ssw.cpp:57-91, ssw_core.cpp:854-1053).
+ * For comparison k of a compare with starts (scores and ranges as returned),
+ * the CIGAR from the start to the end of its alignment: runs (length << 4 |
+ * op), op 0 = M (A and B), 1 = I (A only), 2 = D (B only), written to
+ * cigar[cigar_offsets[k] ..], their count to cigar_len[k] (0 for a score of
+ * 0).  Capacity cigar_offsets[k+1] - cigar_offsets[k] must be at least
+ * (Aend - Astart + 1) + (Bend - Bstart + 1).  A[Astart..Aend] and
+ * B[Bstart..Bend] are aligned globally (their optimum is the local score);
+ * ties prefer the diagonal, then the A gap when strictly better, and a gap
+ * opens only when strictly better than extending.  Host buffers in and out.
+ */
+int ipuma_traceback_local(ipuma_engine *e, int32_t n, const char *a_chars,
+                          const int64_t *a_offsets, const char *b_chars, const int64_t *b_offsets,
+                          const int32_t *scores, const int32_t *a_range, const int32_t *b_range,
+                          uint32_t *cigar, const int64_t *cigar_offsets, int32_t *cigar_len);
+
 /* Batching policy of the device path (replaces IPUAlgoConfig.fillAlgo's
  * partition::fillFirst/roundRobin/greedy, src/partition.cpp:197-462). */
 enum {

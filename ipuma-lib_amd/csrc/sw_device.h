@@ -113,6 +113,28 @@ struct GenericArgs {
   int32_t *b_range;
 };
 
+// Arguments of the traceback kernel (one comparison per lane).
+struct TraceArgs {
+  const uint8_t *codes;
+  const int64_t *a_off;     // [n] start of A in `codes`
+  const int64_t *b_off;     // [n]
+  const int32_t *a_range;   // [n] end << 16 | start
+  const int32_t *b_range;   // [n]
+  const int32_t *scores;    // [n]
+  const int64_t *dir_off;   // [n] byte offset of the comparison's direction cells
+  const int64_t *work_off;  // [n] int64 offset of its row buffers (2 x (|B region| + 1))
+  const int64_t *cig_off;   // [n] offset of its runs in `cigar`
+  int32_t n;
+  int32_t gap_o;            // -(gapInit + gapExtend): the first residue of a gap
+  int32_t gap_x;            // -gapExtend: each further residue
+  int32_t kp;               // table side
+  const int16_t *table;     // kp x kp, row = A code
+  uint8_t *dir;
+  int64_t *work;
+  uint32_t *cigar;          // runs, end to start
+  int32_t *cig_len;         // [n] runs, or -2 when the region's optimum differs from the score
+};
+
 // Arguments of the bucketing pipeline.
 struct BucketArgs {
   int32_t n;
@@ -147,6 +169,7 @@ hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_encode(const uint8_t *chars, int64_t len, uint8_t *codes, const uint8_t *table,
                          hipStream_t st);
 hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st);
+hipError_t launch_traceback(const TraceArgs &T, hipStream_t st);
 hipError_t launch_wire_gather(int64_t nseq, const uint8_t *wire, const int64_t *src,
                               const int64_t *dst, const int32_t *len, const uint8_t *remap,
                               uint8_t *arena, hipStream_t st);

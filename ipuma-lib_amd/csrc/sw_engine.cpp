@@ -147,6 +147,8 @@ struct ipuma_engine {
   // prepared input buffers (the reference's wire format)
   uint8_t *d_wiretab = nullptr;  // code -> code inside the matrix
   DevBuf w_wire, w_desc, w_arena, w_res;
+  // traceback
+  DevBuf t_desc, t_dir, t_work, t_cig;
   std::vector<int64_t> h_src, h_dst;
   std::vector<int32_t> h_len;
   // executor thread of asynchronously submitted prepared batches
@@ -635,6 +637,10 @@ void ipuma_engine_destroy(ipuma_engine *e) {
   e->w_desc.release();
   e->w_arena.release();
   e->w_res.release();
+  e->t_desc.release();
+  e->t_dir.release();
+  e->t_work.release();
+  e->t_cig.release();
   if (e->d_wiretab) (void)hipFree(e->d_wiretab);
   if (e->order_ev) (void)hipEventDestroy(e->order_ev);
   for (auto &ev : e->ev)
@@ -908,6 +914,128 @@ int ipuma_prepared_join(ipuma_engine *e, int64_t job) {
   const PreparedJob j = it->second;
   e->pj_jobs.erase(it);
   if (j.rc) return fail(j.rc, "%s", j.err.c_str());
+  return IPUMA_OK;
+}
+
+int ipuma_traceback_local(ipuma_engine *e, int32_t n, const char *a_chars,
+                          const int64_t *a_offsets, const char *b_chars, const int64_t *b_offsets,
+                          const int32_t *scores, const int32_t *a_range, const int32_t *b_range,
+                          uint32_t *cigar, const int64_t *cigar_offsets, int32_t *cigar_len) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  if (n < 0) return fail(IPUMA_EINVAL, "n < 0");
+  if (n > 0 && (!a_offsets || !b_offsets || !scores || !a_range || !b_range || !cigar ||
+                !cigar_offsets || !cigar_len))
+    return fail(IPUMA_EINVAL, "NULL buffer");
+  std::lock_guard<std::mutex> lock_(e->mu);
+  // regions, capacities, scratch offsets
+  std::vector<const char *> ptrs(2 * (size_t)n);
+  std::vector<int64_t> lens(2 * (size_t)n);
+  std::vector<int64_t> dir_off(n), work_off(n), cig_off(n);
+  int64_t need_cig = 0;
+  for (int32_t k = 0; k < n; ++k) {
+    const int64_t la = a_offsets[k + 1] - a_offsets[k], lb = b_offsets[k + 1] - b_offsets[k];
+    if (la < 0 || lb < 0) return fail(IPUMA_EINVAL, "offsets not ascending at %d", k);
+    if (la > kMaxSeqLen || lb > kMaxSeqLen)
+      return fail(IPUMA_ETOOLONG, "sequence %d longer than %d", k, kMaxSeqLen);
+    ptrs[2 * k] = a_chars + a_offsets[k];
+    ptrs[2 * k + 1] = b_chars + b_offsets[k];
+    lens[2 * k] = la;
+    lens[2 * k + 1] = lb;
+    const uint32_t ar = (uint32_t)a_range[k], br = (uint32_t)b_range[k];
+    const int64_t as = ar & 0xFFFF, ae = ar >> 16, bs = br & 0xFFFF, be = br >> 16;
+    int64_t m = 0, nb = 0;
+    if (scores[k] > 0) {
+      if (ae < as || be < bs || ae >= la || be >= lb)
+        return fail(IPUMA_EINVAL, "comparison %d: ranges outside its sequences", k);
+      m = ae - as + 1;
+      nb = be - bs + 1;
+    }
+    const int64_t cap = cigar_offsets[k + 1] - cigar_offsets[k];
+    if (cap < m + nb)
+      return fail(IPUMA_EINVAL, "comparison %d: CIGAR capacity %lld < %lld", k, (long long)cap,
+                  (long long)(m + nb));
+    dir_off[k] = m * nb;   // sizes for now; turned into offsets per chunk below
+    work_off[k] = 2 * (nb + 1);
+    cig_off[k] = m + nb;
+    need_cig += m + nb;
+  }
+  (void)need_cig;
+  std::vector<int64_t> offs;
+  int64_t total = 0;
+  int rc;
+  if ((rc = stage_arena(e, ptrs, lens, offs, total))) return rc;
+  // chunks of at most 1 GiB of direction cells (one pair may exceed it alone)
+  const int64_t kChunkDir = (int64_t)1 << 30;
+  std::vector<int64_t> h_desc;
+  std::vector<uint32_t> h_cig;
+  std::vector<int32_t> h_len;
+  for (int32_t c0 = 0; c0 < n;) {
+    int32_t c1 = c0;
+    int64_t dsum = 0, wsum = 0, csum = 0;
+    while (c1 < n && (c1 == c0 || dsum + dir_off[c1] <= kChunkDir)) {
+      dsum += dir_off[c1];
+      wsum += work_off[c1];
+      csum += cig_off[c1];
+      ++c1;
+    }
+    const int32_t cn = c1 - c0;
+    // descriptors: a_off, b_off, dir_off, work_off, cig_off (int64) + ranges, scores (int32)
+    h_desc.assign((size_t)cn * 5, 0);
+    int64_t dpos = 0, wpos = 0, cpos = 0;
+    for (int32_t k = 0; k < cn; ++k) {
+      h_desc[k] = offs[2 * (c0 + k)];
+      h_desc[cn + k] = offs[2 * (c0 + k) + 1];
+      h_desc[2 * cn + k] = dpos;
+      h_desc[3 * cn + k] = wpos;
+      h_desc[4 * cn + k] = cpos;
+      dpos += dir_off[c0 + k];
+      wpos += work_off[c0 + k];
+      cpos += cig_off[c0 + k];
+    }
+    if ((rc = e->t_desc.ensure((size_t)cn * (5 * 8 + 3 * 4 + 4))) ||
+        (rc = e->t_dir.ensure((size_t)std::max<int64_t>(dsum, 1))) ||
+        (rc = e->t_work.ensure((size_t)std::max<int64_t>(wsum, 1) * 8)) ||
+        (rc = e->t_cig.ensure((size_t)std::max<int64_t>(csum, 1) * 4)))
+      return rc;
+    int64_t *d64 = e->t_desc.as<int64_t>();
+    int32_t *d32 = reinterpret_cast<int32_t *>(d64 + 5 * (size_t)cn);
+    HIP_TRY(hipMemcpyAsync(d64, h_desc.data(), (size_t)cn * 40, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(d32, a_range + c0, (size_t)cn * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(d32 + cn, b_range + c0, (size_t)cn * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(d32 + 2 * (size_t)cn, scores + c0, (size_t)cn * 4, hipMemcpyHostToDevice,
+                           e->stream));
+    TraceArgs T{};
+    T.codes = e->in_codes.as<uint8_t>();
+    T.a_off = d64; T.b_off = d64 + cn; T.dir_off = d64 + 2 * (size_t)cn;
+    T.work_off = d64 + 3 * (size_t)cn; T.cig_off = d64 + 4 * (size_t)cn;
+    T.a_range = d32; T.b_range = d32 + cn; T.scores = d32 + 2 * (size_t)cn;
+    T.cig_len = d32 + 3 * (size_t)cn;
+    T.n = cn;
+    T.gap_o = -(e->sw.gap_init + e->sw.gap_extend);
+    T.gap_x = -e->sw.gap_extend;
+    T.kp = e->kp; T.table = e->d_table;
+    T.dir = e->t_dir.as<uint8_t>(); T.work = e->t_work.as<int64_t>();
+    T.cigar = e->t_cig.as<uint32_t>();
+    if ((rc = begin_work(e, e->stream))) return rc;
+    HIP_TRY(launch_traceback(T, e->stream));
+    if ((rc = end_work(e, e->stream))) return rc;
+    h_cig.resize((size_t)std::max<int64_t>(csum, 1));
+    h_len.resize(cn);
+    HIP_TRY(hipMemcpyAsync(h_cig.data(), T.cigar, (size_t)csum * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(h_len.data(), T.cig_len, (size_t)cn * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (int32_t k = 0; k < cn; ++k) {
+      const int32_t r = h_len[k];
+      cigar_len[c0 + k] = r;
+      const uint32_t *src = h_cig.data() + h_desc[4 * cn + k];
+      uint32_t *dst = cigar + cigar_offsets[c0 + k];
+      for (int32_t q = 0; q < r; ++q) dst[q] = src[r - 1 - q];  // written end to start
+    }
+    c0 = c1;
+  }
+  for (int32_t k = 0; k < n; ++k)
+    if (cigar_len[k] == -2)
+      return fail(IPUMA_EINVAL, "comparison %d: the ranges do not give the reported score", k);
   return IPUMA_OK;
 }
 

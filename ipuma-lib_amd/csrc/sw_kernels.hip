@@ -1330,6 +1330,115 @@ __global__ __launch_bounds__(256) void k_plain_groups(int32_t n, int32_t per_gro
 }
 
 // ---------------------------------------------------------------------------
+// Traceback: the CIGAR of each comparison between the start and end positions
+// a compare with starts reported.  A[as..ae] and B[bs..be] are aligned
+// globally with affine gaps, rows over A and columns over B; the optimum
+// equals the reported local score (checked), and the traceback prefers the
+// diagonal when it is at least both gap states, then the A-gap state when it
+// is strictly larger, and opens a gap only when opening is strictly better
+// than extending (the rules of SSW's banded_sw, without its band).  One
+// comparison per lane, int64 arithmetic; a direction byte per cell (bits 0-1
+// H source 0 diag / 1 A gap / 2 B gap, bit 2 A gap opened, bit 3 B gap opened).
+// Runs (length << 4 | op, op 0 M, 1 I = A only, 2 D = B only) are written end
+// to start.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_traceback(TraceArgs T) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= T.n) return;
+  const int s = T.scores[p];
+  const uint32_t ar = (uint32_t)T.a_range[p], br = (uint32_t)T.b_range[p];
+  const int as = (int)(ar & 0xFFFF), ae = (int)(ar >> 16);
+  const int bs = (int)(br & 0xFFFF), be = (int)(br >> 16);
+  if (s <= 0 || ae < as || be < bs) {
+    T.cig_len[p] = 0;
+    return;
+  }
+  const int m = ae - as + 1, n = be - bs + 1;
+  const uint8_t *a = T.codes + T.a_off[p] + as;
+  const uint8_t *b = T.codes + T.b_off[p] + bs;
+  uint8_t *dir = T.dir + T.dir_off[p];
+  int64_t *hrow = T.work + T.work_off[p];
+  int64_t *erow = hrow + (n + 1);
+  const int64_t go = T.gap_o, ge = T.gap_x;
+  const int64_t NEG = -((int64_t)1 << 50);
+  hrow[0] = 0;
+  erow[0] = NEG;
+  for (int j = 1; j <= n; ++j) {
+    hrow[j] = -(go + (int64_t)(j - 1) * ge);
+    erow[j] = NEG;
+  }
+  for (int i = 1; i <= m; ++i) {
+    int64_t hdiag = hrow[0];
+    const int64_t h0 = -(go + (int64_t)(i - 1) * ge);
+    hrow[0] = h0;
+    int64_t hleft = h0, f = NEG;
+    const int16_t *row = T.table + (int)a[i - 1] * T.kp;
+    uint8_t *drow = dir + (int64_t)(i - 1) * n;
+    for (int j = 1; j <= n; ++j) {
+      const int64_t hup = hrow[j], eup = erow[j];
+      const bool eo = hup - go > eup - ge;
+      const int64_t e = eo ? hup - go : eup - ge;
+      const bool fo = hleft - go > f - ge;
+      f = fo ? hleft - go : f - ge;
+      const int64_t d = hdiag + row[b[j - 1]];
+      int64_t h = d;
+      uint32_t src = 0;
+      if (e > h || f > h) {
+        src = e > f ? 1u : 2u;
+        h = e > f ? e : f;
+      }
+      drow[j - 1] = (uint8_t)(src | (eo ? 4u : 0u) | (fo ? 8u : 0u));
+      hdiag = hup;
+      hrow[j] = h;
+      erow[j] = e;
+      hleft = h;
+    }
+  }
+  if (hrow[n] != (int64_t)s) {
+    T.cig_len[p] = -2;
+    return;
+  }
+  uint32_t *out = T.cigar + T.cig_off[p];
+  int runs = 0, run_op = -1, run_len = 0;
+  int i = m, j = n, st = 0;
+  while (i > 0 || j > 0) {
+    int op;
+    if (i == 0) {
+      op = 2;
+      --j;
+    } else if (j == 0) {
+      op = 1;
+      --i;
+    } else {
+      const uint32_t dv = dir[(int64_t)(i - 1) * n + (j - 1)];
+      if (st == 0) st = (int)(dv & 3u);
+      if (st == 0) {
+        op = 0;
+        --i;
+        --j;
+      } else if (st == 1) {
+        op = 1;
+        st = (dv & 4u) ? 0 : 1;
+        --i;
+      } else {
+        op = 2;
+        st = (dv & 8u) ? 0 : 2;
+        --j;
+      }
+    }
+    if (op == run_op) {
+      ++run_len;
+    } else {
+      if (run_len) out[runs++] = ((uint32_t)run_len << 4) | (uint32_t)run_op;
+      run_op = op;
+      run_len = 1;
+    }
+  }
+  if (run_len) out[runs++] = ((uint32_t)run_len << 4) | (uint32_t)run_op;
+  T.cig_len[p] = runs;
+}
+
+// ---------------------------------------------------------------------------
 // Prepared input buffers (the reference's wire format): the residues of each
 // slot's A and B are copied from their tile regions to 4-aligned places of a
 // codes arena, codes outside the scoring matrix mapped by `remap` (as the
@@ -1392,6 +1501,12 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
   hipLaunchKernelGGL(k_scan_cost, dim3(1), dim3(1024), 0, st, B.costhist);
   hipLaunchKernelGGL(k_order_groups, dim3(std::min(nblocks(B.max_groups, 256), 512)), dim3(256), 0,
                      st, B.max_groups, B.ctl, B.gtmp, B.gbin, B.grank, B.costhist, B.groups);
+  return hipGetLastError();
+}
+
+hipError_t launch_traceback(const TraceArgs &T, hipStream_t st) {
+  if (T.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_traceback, dim3(nblocks(T.n, 64)), dim3(64), 0, st, T);
   return hipGetLastError();
 }
 

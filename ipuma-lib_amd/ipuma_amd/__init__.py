@@ -172,6 +172,7 @@ def lib():
     L.ipuma_prepared_compare.argtypes = [_P, _P, _i64, _P, _i64, _i32]
     L.ipuma_prepared_submit.argtypes = [_P, _P, _i64, _P, _i64, ctypes.POINTER(_i64)]
     L.ipuma_prepared_join.argtypes = [_P, _i64]
+    L.ipuma_traceback_local.argtypes = [_P, _i32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     return L
 
 
@@ -383,6 +384,28 @@ class SWAlgorithm:
         """blocking_join_prepared_remote_compare (src/ipu_batch_affine.cpp:507-555)."""
         return job.join()
 
+    def traceback(self, A, B, scores, a_range, b_range):
+        """CIGAR runs (length << 4 | op; 0 M, 1 I = A only, 2 D = B only) of each
+        comparison's alignment, from the ranges of a compare with starts."""
+        n = len(A)
+        abuf, aoff = _pack(A)
+        bbuf, boff = _pack(B)
+        sc = np.ascontiguousarray(scores, np.int32)
+        ar = np.ascontiguousarray(a_range, np.int32)
+        br = np.ascontiguousarray(b_range, np.int32)
+        u = ar.astype(np.int64) & 0xFFFFFFFF
+        v = br.astype(np.int64) & 0xFFFFFFFF
+        cap = np.where(sc > 0, ((u >> 16) - (u & 0xFFFF) + 1) + ((v >> 16) - (v & 0xFFFF) + 1), 0)
+        offs = np.zeros(n + 1, np.int64)
+        offs[1:] = np.cumsum(cap)
+        cig = np.zeros(max(int(offs[-1]), 1), np.uint32)
+        lens = np.zeros(n, np.int32)
+        _check(lib().ipuma_traceback_local(self._h, n, abuf, aoff.ctypes.data, bbuf,
+                                           boff.ctypes.data, sc.ctypes.data, ar.ctypes.data,
+                                           br.ctypes.data, cig.ctypes.data, offs.ctypes.data,
+                                           lens.ctypes.data))
+        return [cig[offs[k]:offs[k] + lens[k]].copy() for k in range(n)]
+
     def get_result(self) -> BlockAlignmentResults:
         """get_result (src/ipu_batch_affine.cpp:408): copies sized tilesUsed*maxBatches."""
         r = self._result
@@ -450,6 +473,11 @@ class SWAlgorithm:
 
     def synchronize(self):
         _check(lib().ipuma_synchronize(self._h))
+
+
+def cigar_string(runs) -> str:
+    """'12M1I3M' from CIGAR runs."""
+    return "".join(f"{int(r) >> 4}{'MID'[int(r) & 15]}" for r in runs)
 
 
 def unpack_range(x):

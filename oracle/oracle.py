@@ -63,6 +63,10 @@ def _lib():
         _i32p, _i32p, _i32p]
     lib.oracle_sw_batch.restype = ctypes.c_int
     lib.oracle_max_threads.restype = ctypes.c_int
+    lib.oracle_cigar.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 4 + [
+        ctypes.POINTER(ctypes.c_int8), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
+    lib.oracle_cigar.restype = ctypes.c_int
     return lib
 
 
@@ -78,6 +82,10 @@ def _ref():
     lib.ssw_ref_align_matrix.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
                                          ctypes.c_int, _i8p, ctypes.c_int, _i8p,
                                          ctypes.c_int, ctypes.c_int, _i32p]
+    if hasattr(lib, "ssw_ref_align_matrix_cigar"):
+        lib.ssw_ref_align_matrix_cigar.argtypes = [
+            ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _i8p, ctypes.c_int,
+            _i8p, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_char_p, ctypes.c_int]
     return lib
 
 
@@ -146,6 +154,52 @@ def sw_batch(seqs_a, seqs_b, similarity=SIM_NA, datatype=DT_NA, match=1, mismatc
     mat = select_matrix(similarity, match, mismatch, ambiguity)
     codes, ao, al, bo, bl = arena(seqs_a, seqs_b, datatype)
     return sw_batch_codes(codes, ao, al, bo, bl, mat, gap_init, gap_extend, forward_only, nthreads)
+
+
+def cigar(a_codes, b_codes, a_range: int, b_range: int, score: int, matrix: np.ndarray,
+          gap_init: int, gap_extend: int):
+    """oracle_cigar: runs (length << 4 | op), op 0 = M, 1 = I (A only), 2 = D (B only),
+    of the alignment between the reported start and end positions."""
+    a = np.ascontiguousarray(a_codes, np.uint8)
+    b = np.ascontiguousarray(b_codes, np.uint8)
+    ae, as_ = (a_range >> 16) & 0xFFFF, a_range & 0xFFFF
+    be, bs = (b_range >> 16) & 0xFFFF, b_range & 0xFFFF
+    if score <= 0:
+        return np.zeros(0, np.uint32)
+    cap = (ae - as_ + 1) + (be - bs + 1) + 1
+    out = np.zeros(cap, np.uint32)
+    m = np.ascontiguousarray(matrix, np.int8)
+    n = _lib().oracle_cigar(_ptr(a, _u8p), _ptr(b, _u8p), as_, ae, bs, be,
+                            m.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), m.shape[0],
+                            gap_init, gap_extend, score,
+                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), cap)
+    if n < 0:
+        raise ValueError(f"oracle_cigar failed ({n})")
+    return out[:n].copy()
+
+
+def cigar_string(runs) -> str:
+    return "".join(f"{int(r) >> 4}{'MID'[int(r) & 15]}" for r in runs)
+
+
+def cigar_score(a_codes, b_codes, a_start: int, b_start: int, runs, matrix, gap_init, gap_extend):
+    """Score of the alignment the runs describe (affine gaps)."""
+    go, ge = -(gap_init + gap_extend), -gap_extend
+    i, j, s = a_start, b_start, 0
+    for r in runs:
+        ln, op = int(r) >> 4, int(r) & 15
+        if op == 0:
+            for _ in range(ln):
+                s += int(matrix[a_codes[i], b_codes[j]])
+                i += 1
+                j += 1
+        else:
+            s -= go + (ln - 1) * ge
+            if op == 1:
+                i += ln
+            else:
+                j += ln
+    return s, i - 1, j - 1
 
 
 def max_threads() -> int:
@@ -262,6 +316,22 @@ def ssw_matrix(query: str, ref: str, matrix: np.ndarray, translate: np.ndarray, 
     if rc != 0:
         raise RuntimeError("ssw align failed")
     return tuple(int(x) for x in out)
+
+
+def ssw_matrix_cigar(query: str, ref: str, matrix: np.ndarray, translate: np.ndarray,
+                     gap_open: int, gap_extend: int):
+    """ssw_matrix plus SSW's CIGAR string (query = A, ref = B)."""
+    out = np.zeros(5, np.int32)
+    mat = np.ascontiguousarray(matrix, dtype=np.int8)
+    tr = np.ascontiguousarray(translate, dtype=np.int8)
+    buf = ctypes.create_string_buffer(4 * (len(query) + len(ref)) + 64)
+    rc = _ref().ssw_ref_align_matrix_cigar(query.encode(), len(query), ref.encode(), len(ref),
+                                           _ptr(mat, _i8p), mat.shape[0], _ptr(tr, _i8p),
+                                           gap_open, gap_extend, _ptr(out, _i32p), buf,
+                                           len(buf))
+    if rc != 0:
+        raise RuntimeError("ssw align failed")
+    return tuple(int(x) for x in out), buf.value.decode()
 
 
 def translate_table(datatype: int) -> np.ndarray:

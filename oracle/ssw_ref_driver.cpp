@@ -22,6 +22,11 @@ void fill(const StripedSmithWaterman::Alignment &aln, int32_t out[5]) {
   out[3] = aln.ref_begin;
   out[4] = aln.ref_end;
 }
+void cigar_out(const StripedSmithWaterman::Alignment &aln, char *cigar, int cap) {
+  if (!cigar || cap <= 0) return;
+  std::strncpy(cigar, aln.cigar_string.c_str(), (size_t)cap - 1);
+  cigar[cap - 1] = 0;
+}
 }  // namespace
 
 extern "C" {
@@ -52,6 +57,36 @@ int ssw_ref_align_matrix(const char *query, int qlen, const char *ref, int rlen,
   int mask = qlen / 2 > 15 ? qlen / 2 : 15;
   if (!aligner.Align(query, qlen, ref, rlen, filter, &aln, mask)) return -1;
   fill(aln, out);
+  return 0;
+}
+
+// As ssw_ref_align_dna / ssw_ref_align_matrix, plus SSW's CIGAR string (soft
+// clips, M, I, D; query = A, ref = B) written to `cigar` (cap bytes).
+int ssw_ref_align_dna_cigar(const char *query, int qlen, const char *ref, int rlen, int match,
+                            int mismatch, int gap_open, int gap_extend, int ambiguity,
+                            int32_t out[5], char *cigar, int cap) {
+  StripedSmithWaterman::Aligner aligner(match, mismatch, gap_open, gap_extend, ambiguity);
+  StripedSmithWaterman::Filter filter;
+  StripedSmithWaterman::Alignment aln;
+  int mask = qlen / 2 > 15 ? qlen / 2 : 15;
+  if (!aligner.Align(query, qlen, ref, rlen, filter, &aln, mask)) return -1;
+  fill(aln, out);
+  cigar_out(aln, cigar, cap);
+  return 0;
+}
+
+int ssw_ref_align_matrix_cigar(const char *query, int qlen, const char *ref, int rlen,
+                               const int8_t *matrix, int dim, const int8_t *translate,
+                               int gap_open, int gap_extend, int32_t out[5], char *cigar,
+                               int cap) {
+  StripedSmithWaterman::Aligner aligner(matrix, dim, translate, 128);
+  aligner.SetGapPenalty(gap_open, gap_extend);
+  StripedSmithWaterman::Filter filter;
+  StripedSmithWaterman::Alignment aln;
+  int mask = qlen / 2 > 15 ? qlen / 2 : 15;
+  if (!aligner.Align(query, qlen, ref, rlen, filter, &aln, mask)) return -1;
+  fill(aln, out);
+  cigar_out(aln, cigar, cap);
   return 0;
 }
 }

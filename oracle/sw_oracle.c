@@ -17,6 +17,9 @@
  *                          src/codelets/codelet_batch_affine.cpp:89-115
  *   - range packing (end<<16 | start, uint16 each)
  *                          src/codelets/codelet_batch_affine.cpp:117-123
+ *   - CIGAR between start and end (no counterpart in the IPU path; the
+ *     traceback rules of SSW's banded_sw, ssw_core.cpp:854-1053, without
+ *     its band): oracle_cigar below
  *
  * Definitions where the reference is undefined (documented, deliberate):
  *   - an empty A or B gives score 0 and ranges 0 (the reference vertex
@@ -285,6 +288,107 @@ int oracle_sw_batch(int64_t npairs, const uint8_t *codes,
             free(work);
         }
     }
+    return rc;
+}
+
+/*
+ * CIGAR of one comparison's alignment, between the start and end positions a
+ * compare with starts reported.  The reference's IPU path never produces
+ * one; this is the definition the device traceback is checked against:
+ *
+ *   A[as..ae] (query) and B[bs..be] (ref) are aligned globally with affine
+ *   gaps (a gap of k residues costs go + (k-1)*ge, go = -(gI + gE), ge = -gE;
+ *   src/driver.hpp:47-49), and the optimal score equals the local score the
+ *   compare reported (any path between the two cells is a local alignment
+ *   and the reported one reaches the score).  Rows run over A, columns over
+ *   B (SSW's orientation, extern/include/ssw/ssw_core.cpp:854-1053) and the
+ *   traceback takes, like SSW's: the diagonal when it is at least both gap
+ *   states, else the A-gap state when it is strictly larger, else the B-gap
+ *   state; a gap state opens (back to H) only when opening is strictly
+ *   better than extending.
+ *
+ * Output: runs (length << 4 | op), op 0 = M (A and B), 1 = I (A only),
+ * 2 = D (B only), start to end.  Returns the number of runs, -1 if `cap` is
+ * too small or an allocation fails, -2 if the region's optimum differs from
+ * `score`.
+ */
+int oracle_cigar(const uint8_t *a, const uint8_t *b, int as, int ae, int bs, int be,
+                 const int8_t *mat, int dim, int gI, int gE, int score, uint32_t *out,
+                 int cap) {
+    const int m = ae - as + 1, n = be - bs + 1;
+    if (m <= 0 || n <= 0) return 0;
+    const int go = -(gI + gE), ge = -gE;
+    const long NEG = -(1L << 40);
+    long *H = (long *)malloc(sizeof(long) * (size_t)(m + 1) * (n + 1));
+    long *E = (long *)malloc(sizeof(long) * (size_t)(m + 1) * (n + 1));
+    long *F = (long *)malloc(sizeof(long) * (size_t)(m + 1) * (n + 1));
+    uint8_t *ops = (uint8_t *)malloc((size_t)(m + n) + 1);
+    if (!H || !E || !F || !ops) {
+        free(H); free(E); free(F); free(ops);
+        return -1;
+    }
+#define IX(i, j) ((size_t)(i) * (n + 1) + (j)) /* i, j shifted by one: row 0 / col 0 = before */
+    H[IX(0, 0)] = 0;
+    E[IX(0, 0)] = F[IX(0, 0)] = NEG;
+    for (int i = 1; i <= m; ++i) {
+        H[IX(i, 0)] = -(long)(go + (long)(i - 1) * ge);
+        E[IX(i, 0)] = H[IX(i, 0)];
+        F[IX(i, 0)] = NEG;
+    }
+    for (int j = 1; j <= n; ++j) {
+        H[IX(0, j)] = -(long)(go + (long)(j - 1) * ge);
+        F[IX(0, j)] = H[IX(0, j)];
+        E[IX(0, j)] = NEG;
+    }
+    for (int i = 1; i <= m; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const long e = E[IX(i - 1, j)] - ge, eo = H[IX(i - 1, j)] - go;
+            const long f = F[IX(i, j - 1)] - ge, fo = H[IX(i, j - 1)] - go;
+            E[IX(i, j)] = eo > e ? eo : e;
+            F[IX(i, j)] = fo > f ? fo : f;
+            const long d = H[IX(i - 1, j - 1)] + mat[a[as + i - 1] * dim + b[bs + j - 1]];
+            long h = d;
+            if (E[IX(i, j)] > h) h = E[IX(i, j)];
+            if (F[IX(i, j)] > h) h = F[IX(i, j)];
+            H[IX(i, j)] = h;
+        }
+    int rc = 0;
+    if (H[IX(m, n)] != score) {
+        rc = -2;
+    } else {
+        int i = m, j = n, st = 0, len = 0; /* st: 0 H, 1 E (A gap), 2 F (B gap) */
+        while (i > 0 || j > 0) {
+            if (i == 0) { ops[len++] = 2; --j; continue; }
+            if (j == 0) { ops[len++] = 1; --i; continue; }
+            if (st == 0) {
+                const long d = H[IX(i - 1, j - 1)] + mat[a[as + i - 1] * dim + b[bs + j - 1]];
+                const long e = E[IX(i, j)], f = F[IX(i, j)];
+                if (d >= e && d >= f) { ops[len++] = 0; --i; --j; continue; }
+                st = e > f ? 1 : 2;
+            }
+            if (st == 1) {
+                ops[len++] = 1;
+                st = (H[IX(i - 1, j)] - go > E[IX(i - 1, j)] - ge) ? 0 : 1;
+                --i;
+            } else {
+                ops[len++] = 2;
+                st = (H[IX(i, j - 1)] - go > F[IX(i, j - 1)] - ge) ? 0 : 2;
+                --j;
+            }
+        }
+        /* runs, start to end */
+        int runs = 0;
+        for (int k = len - 1; k >= 0;) {
+            int l = k;
+            while (l >= 0 && ops[l] == ops[k]) --l;
+            if (runs >= cap) { rc = -1; break; }
+            out[runs++] = ((uint32_t)(k - l) << 4) | ops[k];
+            k = l;
+        }
+        if (rc == 0) rc = runs;
+    }
+#undef IX
+    free(H); free(E); free(F); free(ops);
     return rc;
 }
 

@@ -293,3 +293,28 @@ def test_prepared_buffers_emulated(emu_lib):
         jobs[0].joined = False
         jobs[0].join()  # joined twice
     eng.close()
+
+
+def test_traceback_emulated(emu_lib):
+    """CIGARs from the emulated traceback kernel equal the oracle's, on DNA and
+    BLOSUM62 pairs with starts, including zero-score pairs."""
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(41)
+    for sim, dt, letters, gi, ge in ((O.SIM_NA, O.DT_NA, "ACGT", -2, -1),
+                                     (O.SIM_BLOSUM62, O.DT_AA, "ARNDCQEGHILKMFPSTWYV", -10, -1)):
+        A = ["".join(rng.choice(list(letters), int(rng.integers(1, 120)))) for _ in range(20)]
+        B = [("".join(rng.choice(list(letters), int(rng.integers(0, 40)))) + a[3:]
+              + "".join(rng.choice(list(letters), int(rng.integers(0, 40))))) or "A" for a in A]
+        A[0], B[0] = "A", "C" * 5  # score 0 under DNA scoring
+        sw = I.SWConfig(gapInit=gi, gapExtend=ge, similarity="na" if dt == O.DT_NA else "blosum62",
+                        datatype="na" if dt == O.DT_NA else "aa")
+        eng = I.SWAlgorithm(sw, I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+        s, ar, br = eng.compare(A, B)
+        got = eng.traceback(A, B, s, ar, br)
+        mat = O.select_matrix(sim)
+        for k in range(len(A)):
+            a, b = O.encode(dt, A[k]), O.encode(dt, B[k])
+            exp = O.cigar(a, b, int(ar[k]), int(br[k]), int(s[k]), mat, gi, ge)
+            assert np.array_equal(got[k], exp), (k, I.cigar_string(got[k]), O.cigar_string(exp))
+        eng.close()

@@ -300,6 +300,40 @@ def test_prepared_buffers(gpu):
     eng.close()
 
 
+def test_traceback(gpu):
+    # CIGARs of a compare with starts, bit-exact against the oracle: DNA and
+    # BLOSUM62 batches and a few long pairs (one lane per comparison, chunked
+    # direction storage)
+    note("traceback: 3000 DNA + 1500 protein pairs + long pairs, CIGAR vs oracle")
+    rng = np.random.default_rng(29)
+
+    def strings(n, letters, lo, hi):
+        return ["".join(rng.choice(letters, int(rng.integers(lo, hi)))) for _ in range(n)]
+
+    dna, aa = list("ACGT"), list("ARNDCQEGHILKMFPSTWYV")
+    A = strings(3000, dna, 30, 250)
+    B = [strings(1, dna, 0, 50)[0] + a[5:] + strings(1, dna, 0, 30)[0] for a in A]
+    for L in (3000, 5000):
+        a = strings(1, dna, L, L + 1)[0]
+        A.append(a)
+        B.append(a[:L // 3] + "ACGTTA" + a[L // 3:2 * L // 3] + a[2 * L // 3 + 7:])
+    PA = strings(1500, aa, 40, 400)
+    PB = [a[: len(a) // 2] + strings(1, aa, 0, 5)[0] + a[len(a) // 2:] for a in PA]
+    for sw, a, b, sim, dt in ((I.SWConfig(gapInit=-2, gapExtend=-1), A, B, O.SIM_NA, O.DT_NA),
+                              (I.SWConfig(gapInit=-10, gapExtend=-1, similarity="blosum62",
+                                          datatype="aa"), PA, PB, O.SIM_BLOSUM62, O.DT_AA)):
+        eng = I.SWAlgorithm(sw, I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+        s, ar, br = eng.compare(a, b)
+        got = eng.traceback(a, b, s, ar, br)
+        mat = O.select_matrix(sim)
+        for k in range(len(a)):
+            exp = O.cigar(O.encode(dt, a[k]), O.encode(dt, b[k]), int(ar[k]), int(br[k]),
+                          int(s[k]), mat, sw.gapInit, sw.gapExtend)
+            assert np.array_equal(got[k], exp), (sw.datatype, k, I.cigar_string(got[k]),
+                                                 O.cigar_string(exp))
+        eng.close()
+
+
 def test_int16_saturation_rerun(gpu):
     # W^L vs W^L under BLOSUM62 scores 11*L: 45056 at L = 4096 overflows int16
     note("int16 saturation -> int32 re-run (W^4096)")
