@@ -88,10 +88,33 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
 // which orders (max H, min row, min column) inside the pair.  Rows merge in
 // order with a strict >, lanes merge at the end by (max H, min row).
 // ---------------------------------------------------------------------------
+// Issue priority among the waves of one SIMD.  By default the oldest wave
+// wins every issue conflict, so the youngest wave on a SIMD barely advances
+// the group it took at launch and finishes it alone, after the queue has run
+// dry.  Instead each wave publishes the steps left in its group and ranks
+// itself among the waves of its SIMD: the most work left issues first, the
+// groups on a SIMD finish together, and the queue's tail stays short.
+struct SimdShare {
+  int *left;   // LDS, per wave slot of the block: steps left (-1: none)
+  int self;    // the wave's slot
+  int p1, p2;  // the other slots on the same SIMD (a slot that stays -1 if fewer)
+};
+
+__device__ __forceinline__ void share_simd(const SimdShare &S, int left, int lane) {
+  if (lane == 0)
+    __hip_atomic_store(&S.left[S.self], left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  const int o1 = __hip_atomic_load(&S.left[S.p1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  const int o2 = __hip_atomic_load(&S.left[S.p2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  const int below = __builtin_amdgcn_readfirstlane((o1 < left ? 1 : 0) + (o2 < left ? 1 : 0));
+  if (below == 0) __builtin_amdgcn_s_setprio(0);
+  else if (below == 1) __builtin_amdgcn_s_setprio(1);
+  else __builtin_amdgcn_s_setprio(2);
+}
+
 template <int MODE, int KP>
 __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__restrict__ tab,
                                           const GroupDesc gd, uint32_t *__restrict__ wscr,
-                                          int lane, int lg_rt) {
+                                          int lane, int lg_rt, const SimdShare &share) {
   constexpr int R = kRows;
   constexpr uint32_t PADC = KP - 1;
   constexpr uint32_t COLSTRIDE = KP * 128;  // bytes of one A-code row of the LDS table
@@ -184,6 +207,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     }
     const int T = mpad + L - 1;
     for (int t0 = 0; t0 < T; t0 += 4) {
+      if ((t0 & 7) == 0) share_simd(share, (nss - ss) * T - t0, lane);
       const uint32_t wa_lo = na_lo, wa_hi = na_hi;
       if (first) {
         const int tn = t0 + 4;
@@ -719,6 +743,8 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   constexpr int WPB = kFastThreads / 64;
   __shared__ uint32_t tab[KP * KP * 32];
   __shared__ BandState band;
+  __shared__ int left[WPB + 1];
+  __shared__ int simd_of[WPB];
   const int lane = threadIdx.x & 63;
   const int ng = min(A.ctl->ngroups, A.max_groups);
   if (ng <= 0) return;  // uniform over the grid
@@ -729,13 +755,25 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
     band.cur = kBandSlots;
     band.free_mask = (1 << kBandSlots) - 1;
   }
+  if (threadIdx.x <= WPB) left[threadIdx.x] = -1;
+  if (lane == 0)  // SIMD of the wave: HW_ID bits 5:4
+    simd_of[threadIdx.x >> 6] = (__builtin_amdgcn_s_getreg((31 << 11) | 4) >> 4) & 3;
   for (int i = threadIdx.x; i < KP * KP * 32; i += blockDim.x)
     tab[i] = (uint32_t)(uint16_t)A.table[i >> 5];
   __syncthreads();
   // band groups first (they are the longest), one block each, then the rest
   if (nband > 0) band_phase<KP>(A, tab, band, lane, nband);
-  const int wave = (int)blockIdx.x * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int wslot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int wave = (int)blockIdx.x * WPB + wslot;
   uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
+  SimdShare share{left, wslot, WPB, WPB};
+  for (int i = 0; i < WPB; ++i)
+    if (i != wslot && simd_of[i] == simd_of[wslot]) {
+      if (share.p1 == WPB) share.p1 = i;
+      else if (share.p2 == WPB) share.p2 = i;
+    }
+  share.p1 = __builtin_amdgcn_readfirstlane(share.p1);
+  share.p2 = __builtin_amdgcn_readfirstlane(share.p2);
   for (int gi = next_group(&A.ctl->queue, nband, ng, lane); gi >= 0;
        gi = next_group(&A.ctl->queue, nband, ng, lane)) {
     const GroupDesc gd = A.groups[gi];
@@ -749,13 +787,16 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
     if (!ok) {
       atomicOr(&A.ctl->error, 4);  // every lane: no lane-conditional branch here either
     } else if (cls == 0) {
-      run_group<kModeSolo, KP>(A, tab, gd, wscr, lane, 0);
+      run_group<kModeSolo, KP>(A, tab, gd, wscr, lane, 0, share);
     } else if (cls == kNumClasses - 1) {
-      run_group<kModeWave, KP>(A, tab, gd, wscr, lane, 6);
+      run_group<kModeWave, KP>(A, tab, gd, wscr, lane, 6, share);
     } else {
-      run_group<kModeRow, KP>(A, tab, gd, wscr, lane, lg);
+      run_group<kModeRow, KP>(A, tab, gd, wscr, lane, lg, share);
     }
   }
+  if (lane == 0)
+    __hip_atomic_store(&left[wslot], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __builtin_amdgcn_s_setprio(0);
 }
 
 template __global__ void k_fast<18>(FastArgs);

@@ -203,6 +203,8 @@ template <class T, class V> inline T hmax(T *p, V v) {
 #define __hip_atomic_fetch_max(p, v, o, s) emu::hmax((p), (v))
 #define __builtin_amdgcn_s_sleep(n) std::this_thread::yield()
 #define __builtin_amdgcn_s_setprio(n) ((void)0)
+// HW_ID: the wave's SIMD in bits 5:4 (slot % 4, as the hardware spreads a block)
+#define __builtin_amdgcn_s_getreg(r) ((int)(((emu::ctx.tid.x >> 6) & 3) << 4))
 
 inline int max(int a, int b) { return a > b ? a : b; }
 inline int min(int a, int b) { return a < b ? a : b; }
