@@ -6,7 +6,7 @@ namespace ipuma {
 
 // Rows of one lane's strip held in registers by the packed kernel.
 constexpr int kRows = 16;
-// Threads per block of the packed kernel (one block per CU: the replicated
+// Threads per block of the packed kernel (one block per CU: the row-pair
 // scoring table takes most of the LDS).
 constexpr int kFastThreads = 768;
 // Lanes-per-pair classes of the packed kernel (pipeline depth inside a wave);

@@ -82,6 +82,14 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
 // and stay strictly below, or tie later in B-major order than, the real cell
 // they came from: they never move the maximum or its first position.
 //
+// Substitution scores come from a row-pair table in LDS: entry (a, b1, b2)
+// holds S[a][b1] in its low and S[a][b2] in its high half, so one 32-bit load
+// serves rows 2q and 2q+1 of one comparison, and two loads plus two byte
+// permutes give both halves' scores of a row pair.  The table (KP^3 words,
+// 70 KB for the 26-letter protein alphabet) is not replicated: its loads hit
+// random banks, which costs less than the address arithmetic a per-row
+// lookup needs.
+//
 // Position tracking: per row, key = H << 16 | (0xFFFF - j); the max key is the
 // row maximum at its first column.  Pipelines of up to 16 lanes share one key
 // between rows 2q and 2q+1: H << 16 | (row 2q ? 1 : 0) << 15 | (0x7FFF - j),
@@ -117,7 +125,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
                                           int lane, int lg_rt, const SimdShare &share) {
   constexpr int R = kRows;
   constexpr uint32_t PADC = KP - 1;
-  constexpr uint32_t COLSTRIDE = KP * 128;  // bytes of one A-code row of the LDS table
+  constexpr uint32_t COLSTRIDE = KP * KP * 4;  // bytes of one A-code row of the LDS table
   // pipeline depth L = 2^lg: fixed for the solo and whole-wave modes
   const int lg = MODE == kModeSolo ? 0 : MODE == kModeWave ? 6 : lg_rt;
   // Pipelines of up to 16 lanes have |A| <= 16384 (the wave scratch holds
@@ -134,7 +142,6 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   const bool last = (k == L - 1);
   const int mpad = gd.mpad;
   const int nss = gd.nss;
-  const uint32_t bank4 = (uint32_t)(lane & 31) * 4u;
 
   const int p_lo = A.sorted[gd.base + g];
   const int p_hi = A.sorted[gd.base + NG + g];
@@ -163,7 +170,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     // read back by lane 0: complete those stores and drop stale L1 lines.
     if (ss > 0) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     const int row0 = ((ss << lg) + k) * R;
-    uint32_t rp[R];
+    uint32_t rp[R / 2];  // row-pair table offsets: low half low comparison, high half high comparison
     {
       uint32_t wl[R / 4], wh[R / 4];
       const uint32_t *bl = (const uint32_t *)(b_lo + row0);
@@ -173,14 +180,15 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         wl[q] = (row0 + 4 * q < n_lo) ? bl[q] : 0u;
         wh[q] = (row0 + 4 * q < n_hi) ? bh[q] : 0u;
       }
+      uint32_t cl[R], ch[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const uint32_t cl =
-            (row0 + r < n_lo) ? min((wl[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
-        const uint32_t ch =
-            (row0 + r < n_hi) ? min((wh[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
-        rp[r] = (cl * 128u) | ((ch * 128u) << 16);
+        cl[r] = (row0 + r < n_lo) ? min((wl[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
+        ch[r] = (row0 + r < n_hi) ? min((wh[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
       }
+#pragma unroll
+      for (int q = 0; q < R / 2; ++q)
+        rp[q] = ((cl[2 * q] * KP + cl[2 * q + 1]) * 4u) | (((ch[2 * q] * KP + ch[2 * q + 1]) * 4u) << 16);
     }
     uint32_t H[R], E[R], KL[KN], KH[KN];
 #pragma unroll
@@ -233,19 +241,20 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         // (even) row's bit 15 over a 15-bit 0x7FFF - j
         const uint32_t jinv = PAIRKEY ? ((0x7FFFu - (uint32_t)j) & 0x7FFFu) | 0x8000u
                                       : 0xFFFFu - (uint32_t)j;
-        const char *tl = (const char *)tab + (col_lo + bank4);
-        const char *th = (const char *)tab + (col_hi + bank4);
-        // keep the row parts packed two per register across the loop (pinned
-        // once per step, ahead of the loads, so the 2*R loads can issue
+        const char *tl = (const char *)tab + col_lo;
+        const char *th = (const char *)tab + col_hi;
+        // keep the row-pair offsets packed two per register across the loop
+        // (pinned once per step, ahead of the loads, so the R loads can issue
         // back to back and their latency overlaps)
 #pragma unroll
-        for (int r = 0; r < R; ++r) asm volatile("" : "+v"(rp[r]));
+        for (int q = 0; q < R / 2; ++q) asm volatile("" : "+v"(rp[q]));
         uint32_t sv[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const uint32_t slo = *(const uint16_t *)(tl + (rp[r] & 0xFFFFu));
-          const uint32_t shi = *(const uint16_t *)(th + (rp[r] >> 16));
-          sv[r] = __builtin_amdgcn_perm(shi, slo, 0x05040100u);
+        for (int q = 0; q < R / 2; ++q) {
+          const uint32_t wlo = *(const uint32_t *)(tl + (rp[q] & 0xFFFFu));
+          const uint32_t whi = *(const uint32_t *)(th + (rp[q] >> 16));
+          sv[2 * q] = __builtin_amdgcn_perm(whi, wlo, 0x05040100u);
+          sv[2 * q + 1] = __builtin_amdgcn_perm(whi, wlo, 0x07060302u);
         }
         // t2 of row r is diag + s; the old H of row r is consumed (as the
         // diagonal of row r + 1) before the new H of row r is written.
@@ -428,7 +437,7 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
                            int sl, int s, int lane) {
   constexpr int R = kRows;
   constexpr uint32_t PADC = KP - 1;
-  constexpr uint32_t COLSTRIDE = KP * 128;
+  constexpr uint32_t COLSTRIDE = KP * KP * 4;
   BandSlot &S = B.slot[sl];
   const int k = lane;
   const bool first = (k == 0);
@@ -436,7 +445,6 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
   const int mpad = __builtin_amdgcn_readfirstlane(S.gd.mpad);
   const int nss = __builtin_amdgcn_readfirstlane(S.gd.nss);
   const int gbase = __builtin_amdgcn_readfirstlane(S.gd.base);
-  const uint32_t bank4 = (uint32_t)(lane & 31) * 4u;
   uint32_t *__restrict__ bscr =
       A.band_scratch + ((int64_t)blockIdx.x * kBandSlots + sl) * kWaveScratchWords;
 
@@ -461,7 +469,7 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
   const uint32_t gap_o = A.gap_o, gap_x = A.gap_x;
 
   const int row0 = ((s << 6) + k) * R;
-  uint32_t rp[R];
+  uint32_t rp[R / 2];  // row-pair table offsets: low half low comparison, high half high comparison
   {
     uint32_t wl[R / 4], wh[R / 4];
     const uint32_t *bl = (const uint32_t *)(b_lo + row0);
@@ -471,14 +479,15 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
       wl[q] = (row0 + 4 * q < n_lo) ? bl[q] : 0u;
       wh[q] = (row0 + 4 * q < n_hi) ? bh[q] : 0u;
     }
+    uint32_t cl[R], ch[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const uint32_t cl =
-          (row0 + r < n_lo) ? min((wl[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
-      const uint32_t ch =
-          (row0 + r < n_hi) ? min((wh[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
-      rp[r] = (cl * 128u) | ((ch * 128u) << 16);
+      cl[r] = (row0 + r < n_lo) ? min((wl[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
+      ch[r] = (row0 + r < n_hi) ? min((wh[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
     }
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q)
+      rp[q] = ((cl[2 * q] * KP + cl[2 * q + 1]) * 4u) | (((ch[2 * q] * KP + ch[2 * q + 1]) * 4u) << 16);
   }
   uint32_t H[R], E[R], KL[R], KH[R];
 #pragma unroll
@@ -535,16 +544,17 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
       col_hi = from_prev<kModeWave>(ac_hi * COLSTRIDE, col_hi, first);
       const int j = t - k;
       const uint32_t jinv = 0xFFFFu - (uint32_t)j;
-      const char *tl = (const char *)tab + (col_lo + bank4);
-      const char *th = (const char *)tab + (col_hi + bank4);
+      const char *tl = (const char *)tab + col_lo;
+      const char *th = (const char *)tab + col_hi;
 #pragma unroll
-      for (int r = 0; r < R; ++r) asm volatile("" : "+v"(rp[r]));
+      for (int q = 0; q < R / 2; ++q) asm volatile("" : "+v"(rp[q]));
       uint32_t sv[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const uint32_t slo = *(const uint16_t *)(tl + (rp[r] & 0xFFFFu));
-        const uint32_t shi = *(const uint16_t *)(th + (rp[r] >> 16));
-        sv[r] = __builtin_amdgcn_perm(shi, slo, 0x05040100u);
+      for (int q = 0; q < R / 2; ++q) {
+        const uint32_t wlo = *(const uint32_t *)(tl + (rp[q] & 0xFFFFu));
+        const uint32_t whi = *(const uint32_t *)(th + (rp[q] >> 16));
+        sv[2 * q] = __builtin_amdgcn_perm(whi, wlo, 0x05040100u);
+        sv[2 * q + 1] = __builtin_amdgcn_perm(whi, wlo, 0x07060302u);
       }
       s16x2 t2 = __builtin_elementwise_add_sat(S2(hdiag), S2(sv[0]));
       hdiag = rh;
@@ -741,7 +751,7 @@ __device__ __forceinline__ int next_group(int32_t *queue, int first, int ng, int
 template <int KP>
 __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   constexpr int WPB = kFastThreads / 64;
-  __shared__ uint32_t tab[KP * KP * 32];
+  __shared__ uint32_t tab[KP * KP * KP];
   __shared__ BandState band;
   __shared__ int left[WPB + 1];
   __shared__ int simd_of[WPB];
@@ -758,8 +768,12 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   if (threadIdx.x <= WPB) left[threadIdx.x] = -1;
   if (lane == 0)  // SIMD of the wave: HW_ID bits 5:4
     simd_of[threadIdx.x >> 6] = (__builtin_amdgcn_s_getreg((31 << 11) | 4) >> 4) & 3;
-  for (int i = threadIdx.x; i < KP * KP * 32; i += blockDim.x)
-    tab[i] = (uint32_t)(uint16_t)A.table[i >> 5];
+  // row-pair table: word (a * KP + b1) * KP + b2 = S[a][b1] | S[a][b2] << 16
+  for (int i = threadIdx.x; i < KP * KP * KP; i += blockDim.x) {
+    const int a = i / (KP * KP), b1 = (i / KP) % KP, b2 = i % KP;
+    tab[i] = (uint32_t)(uint16_t)A.table[a * KP + b1] |
+             ((uint32_t)(uint16_t)A.table[a * KP + b2] << 16);
+  }
   __syncthreads();
   // band groups first (they are the longest), one block each, then the rest
   if (nband > 0) band_phase<KP>(A, tab, band, lane, nband);
