@@ -38,6 +38,9 @@ __device__ __forceinline__ uint32_t W(s16x2 v) { return __builtin_bit_cast(uint3
 // cross-lane moves), 2..16 lanes (DPP row_shr:1 inside rows of 16 lanes), and
 // the whole wave (L = 64, DPP wave_shr:1).
 constexpr int kModeSolo = 0;
+// Longest padded |A| of a whole-wave group with row-pair keys: its columns
+// run to mpad + 62 and must fit 15 bits.
+constexpr int kPairKeyMaxM = 32768 - 64;
 constexpr int kModeRow = 1;
 constexpr int kModeWave = 2;
 
@@ -49,9 +52,10 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
     (void)v; (void)first;
     return own;
   } else if constexpr (MODE == kModeWave) {
-    // DPP wave_shr:1: lane i takes lane i-1 across the whole wave
-    const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x138, 0xf, 0xf, false);
-    return first ? own : r;
+    // DPP wave_shr:1: lane i takes lane i-1 across the whole wave; lane 0 has
+    // no source and keeps `old` (bound_ctrl off), which is `own`
+    (void)first;
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x138, 0xf, 0xf, false);
   } else {
     const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x111, 0xf, 0xf, false);
     return first ? own : r;
@@ -119,7 +123,7 @@ __device__ __forceinline__ void share_simd(const SimdShare &S, int left, int lan
   else __builtin_amdgcn_s_setprio(2);
 }
 
-template <int MODE, int KP>
+template <int MODE, int KP, bool PAIRKEY>
 __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__restrict__ tab,
                                           const GroupDesc gd, uint32_t *__restrict__ wscr,
                                           int lane, int lg_rt, const SimdShare &share) {
@@ -128,11 +132,11 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   constexpr uint32_t COLSTRIDE = KP * KP * 4;  // bytes of one A-code row of the LDS table
   // pipeline depth L = 2^lg: fixed for the solo and whole-wave modes
   const int lg = MODE == kModeSolo ? 0 : MODE == kModeWave ? 6 : lg_rt;
-  // Pipelines of up to 16 lanes have |A| <= 16384 (the wave scratch holds
-  // mpad * 64/L <= 65536 words), so 15 bits hold the column and one key
-  // covers a row pair; the whole-wave pipeline keeps one key per row with
-  // 16-bit columns (|A| up to 65535).
-  constexpr bool PAIRKEY = MODE != kModeWave;
+  // Row-pair keys hold 15-bit columns: every column a lane computes (up to
+  // mpad + L - 2) must stay below 32768.  Pipelines of up to 16 lanes have
+  // |A| <= 16384 (the wave scratch holds mpad * 64/L <= 65536 words); the
+  // whole-wave pipeline takes them up to mpad = kPairKeyMaxM and keeps one
+  // key per row with 16-bit columns beyond (|A| up to 65535).
   constexpr int KN = PAIRKEY ? R / 2 : R;
   const int L = 1 << lg;
   const int NG = 64 >> lg;  // pipelines per wave
@@ -432,7 +436,7 @@ __device__ __forceinline__ int band_wait(int *prog, int need) {
 
 // Super-strip s of the band group in slot `sl`: the whole-wave pipeline body
 // of run_group, with the boundary row shared through the slot's buffer.
-template <int KP>
+template <int KP, bool PAIRKEY>
 __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
                            int sl, int s, int lane) {
   constexpr int R = kRows;
@@ -489,13 +493,19 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
     for (int q = 0; q < R / 2; ++q)
       rp[q] = ((cl[2 * q] * KP + cl[2 * q + 1]) * 4u) | (((ch[2 * q] * KP + ch[2 * q + 1]) * 4u) << 16);
   }
-  uint32_t H[R], E[R], KL[R], KH[R];
+  // keys as in run_group: one per row pair (15-bit columns) up to
+  // mpad = kPairKeyMaxM, one per row (16-bit columns) beyond
+  constexpr int KN = PAIRKEY ? R / 2 : R;
+  uint32_t H[R], E[R], KL[KN], KH[KN];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     H[r] = 0u;
     E[r] = 0u;
-    KL[r] = 0u;
-    KH[r] = 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < KN; ++q) {
+    KL[q] = 0u;
+    KH[q] = 0u;
   }
   uint32_t out_h = 0u, out_f = 0u, hdiag = 0u;
   uint32_t col_lo = PADC * COLSTRIDE, col_hi = PADC * COLSTRIDE;
@@ -543,7 +553,8 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
       col_lo = from_prev<kModeWave>(ac_lo * COLSTRIDE, col_lo, first);
       col_hi = from_prev<kModeWave>(ac_hi * COLSTRIDE, col_hi, first);
       const int j = t - k;
-      const uint32_t jinv = 0xFFFFu - (uint32_t)j;
+      const uint32_t jinv = PAIRKEY ? ((0x7FFFu - (uint32_t)j) & 0x7FFFu) | 0x8000u
+                                    : 0xFFFFu - (uint32_t)j;
       const char *tl = (const char *)tab + col_lo;
       const char *th = (const char *)tab + col_hi;
 #pragma unroll
@@ -559,6 +570,9 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
       s16x2 t2 = __builtin_elementwise_add_sat(S2(hdiag), S2(sv[0]));
       hdiag = rh;
       uint32_t f = rf;
+      uint32_t kl_even = 0u, kh_even = 0u;
+      uint32_t jr_even = jinv, jr_odd = jinv & 0x7FFFu;
+      if constexpr (PAIRKEY) asm volatile("" : "+v"(jr_even), "+v"(jr_odd));
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         s16x2 t2n = t2;
@@ -569,8 +583,21 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
         const u16x2 ho = __builtin_elementwise_sub_sat(U2(hn), U2(gap_o));
         E[r] = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))));
         f = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(f), U2(gap_x))));
-        KL[r] = max(KL[r], __builtin_amdgcn_perm(hn, jinv, 0x05040100u));
-        KH[r] = max(KH[r], __builtin_amdgcn_perm(hn, jinv, 0x07060100u));
+        if constexpr (PAIRKEY) {
+          const uint32_t jr = (r & 1) ? jr_odd : jr_even;
+          const uint32_t kl = (hn << 16) | jr;
+          const uint32_t kh = (hn & 0xFFFF0000u) | jr;
+          if (r & 1) {
+            KL[r >> 1] = max(max(KL[r >> 1], kl_even), kl);
+            KH[r >> 1] = max(max(KH[r >> 1], kh_even), kh);
+          } else {
+            kl_even = kl;
+            kh_even = kh;
+          }
+        } else {
+          KL[r] = max(KL[r], __builtin_amdgcn_perm(hn, jinv, 0x05040100u));
+          KH[r] = max(KH[r], __builtin_amdgcn_perm(hn, jinv, 0x07060100u));
+        }
         t2 = t2n;
       }
       out_h = H[R - 1];
@@ -591,18 +618,28 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
 
   int bh_lo = 0, bi_lo = 0, bj_lo = 0, bh_hi = 0, bi_hi = 0, bj_hi = 0;
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
+  for (int q = 0; q < KN; ++q) {
     const int hl = (int)(KL[q] >> 16);
     if (hl > bh_lo) {
       bh_lo = hl;
-      bi_lo = row0 + q;
-      bj_lo = 0xFFFF - (int)(KL[q] & 0xFFFFu);
+      if constexpr (PAIRKEY) {
+        bi_lo = row0 + 2 * q + ((KL[q] & 0x8000u) ? 0 : 1);
+        bj_lo = 0x7FFF - (int)(KL[q] & 0x7FFFu);
+      } else {
+        bi_lo = row0 + q;
+        bj_lo = 0xFFFF - (int)(KL[q] & 0xFFFFu);
+      }
     }
     const int hh = (int)(KH[q] >> 16);
     if (hh > bh_hi) {
       bh_hi = hh;
-      bi_hi = row0 + q;
-      bj_hi = 0xFFFF - (int)(KH[q] & 0xFFFFu);
+      if constexpr (PAIRKEY) {
+        bi_hi = row0 + 2 * q + ((KH[q] & 0x8000u) ? 0 : 1);
+        bj_hi = 0x7FFF - (int)(KH[q] & 0x7FFFu);
+      } else {
+        bi_hi = row0 + q;
+        bj_hi = 0xFFFF - (int)(KH[q] & 0xFFFFu);
+      }
     }
   }
   for (int off = 1; off < 64; off <<= 1) {
@@ -722,7 +759,10 @@ __device__ void band_phase(const FastArgs &A, const uint32_t *__restrict__ tab, 
         // The longest band groups bound the launch: their strips issue ahead
         // of the other waves on the SIMD (the other waves fill the gaps).
         if (nss >= kBandPrioStrips) __builtin_amdgcn_s_setprio(2);
-        band_strip<KP>(A, tab, B, sl, nx, lane);
+        if (__builtin_amdgcn_readfirstlane(S.gd.mpad) <= kPairKeyMaxM)
+          band_strip<KP, true>(A, tab, B, sl, nx, lane);
+        else
+          band_strip<KP, false>(A, tab, B, sl, nx, lane);
         __builtin_amdgcn_s_setprio(0);
       } else {
         band_load(A, B, cur, lane, nband);
@@ -801,11 +841,14 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
     if (!ok) {
       atomicOr(&A.ctl->error, 4);  // every lane: no lane-conditional branch here either
     } else if (cls == 0) {
-      run_group<kModeSolo, KP>(A, tab, gd, wscr, lane, 0, share);
+      run_group<kModeSolo, KP, true>(A, tab, gd, wscr, lane, 0, share);
     } else if (cls == kNumClasses - 1) {
-      run_group<kModeWave, KP>(A, tab, gd, wscr, lane, 6, share);
+      if (gd.mpad <= kPairKeyMaxM)
+        run_group<kModeWave, KP, true>(A, tab, gd, wscr, lane, 6, share);
+      else
+        run_group<kModeWave, KP, false>(A, tab, gd, wscr, lane, 6, share);
     } else {
-      run_group<kModeRow, KP>(A, tab, gd, wscr, lane, lg, share);
+      run_group<kModeRow, KP, true>(A, tab, gd, wscr, lane, lg, share);
     }
   }
   if (lane == 0)

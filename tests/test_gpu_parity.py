@@ -232,6 +232,27 @@ def test_band_groups(gpu):
         assert eng.stats()["band_groups"] >= (3500 if lanes else 256), eng.stats()
 
 
+def test_wide_a_key_layouts(gpu):
+    # whole-wave groups keep row-pair keys (15-bit columns) while |A| <= 32704
+    # and one key per row (16-bit columns) beyond; the best cell sits near the
+    # end of A, so its column uses the top bits of either layout.  |B| = 200
+    # runs one super-strip per wave, |B| = 1,500 a band group.
+    note("wide A: |A| 32704 / 32705 / 40000 x |B| 200 / 1500, with starts")
+    rng = np.random.default_rng(23)
+    pairs = []
+    for m in (32704, 32705, 40000):
+        for nb in (200, 1500):
+            a = W._dna_draw(rng, m)
+            b = W._mutate(rng, a[m - nb - 5: m - 5], 0.08, 0.01, W._dna_draw, W._dna_subst)
+            pairs.append((a, b))
+    batch = W.make_batch(pairs, "wide", W.DNA_SW)
+    eng = engine_for(batch["sw"], False)
+    got = device_run(eng, batch)
+    assert_same(got, oracle_batch(batch, False), "wide A")
+    assert eng.stats()["band_groups"] >= 1, eng.stats()
+    assert min(got[1].astype(np.uint32) >> 16) > 30000  # end columns in the top bits
+
+
 def test_prepared_buffers(gpu):
     # the reference's wire format end to end: prepare_remote with each fill
     # algorithm, prepared_remote_compare, transferResults; a comparison-list
