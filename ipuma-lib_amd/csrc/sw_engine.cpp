@@ -3,8 +3,8 @@
 // One engine = one HIP device + one stream + the scoring table in HBM + grow-
 // only scratch.  A device batch is a fixed chain of kernels on one stream,
 // with no host round trip:
-//   bucketing (k_sum_cells, k_classify, k_scan_bins, k_scatter, k_groups,
-//   k_scan_cost, k_order_groups)  ->  k_fast (packed int16, persistent)  ->
+//   bucketing (k_sum_cells, k_classify, k_scan_run, k_scan_runs, k_scatter,
+//   k_groups, k_scan_cost, k_order_groups)  ->  k_fast (packed int16, persistent)  ->
 //   k_generic32 (exact int32 re-run of saturated comparisons)  ->
 //   [k_reverse_prefix -> same chain on the reversed prefixes -> k_finish_starts]
 // The last block only runs without forward_only, for start positions.

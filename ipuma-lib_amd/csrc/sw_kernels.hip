@@ -1120,86 +1120,72 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
   if (tid < kNumClasses && s_count[tid]) atomicAdd(&ctl->class_count[tid], s_count[tid]);
 }
 
-// Single block: exclusive scan of the bins with every run (class, super-strip
-// bin) starting at a multiple of kRunAlign slots, and the first group of each
-// run.  Thread t owns the 128 bins [128 t, 128 t + 128): half of run t / 2.
-constexpr int kScanThreads = kNumBins / 128;
-static_assert(kNumBins % 128 == 0 && kMBins == 256 && kScanThreads <= 1024, "scan layout");
-static_assert(kRuns <= kScanThreads && kRuns % 64 == 0, "one thread per run");
-__global__ __launch_bounds__(kScanThreads) void k_scan_bins(int32_t *hist, BucketCtl *ctl) {
-  __shared__ int32_t part[kScanThreads];
-  __shared__ int32_t rstart[kRuns];
+// Scan of the bins in two kernels.  k_scan_run: one block per run (class,
+// super-strip bin) of kMBins consecutive bins turns each bin's count into
+// its offset inside the run and leaves the run's total in run_start.
+// k_scan_runs: one thread per run gives every run its start (a multiple of
+// kRunAlign slots) and its first group.  k_scatter adds the two.
+static_assert(kMBins == 256 && kRuns % 64 == 0 && kRuns <= 1024, "scan layout");
+__global__ __launch_bounds__(kMBins) void k_scan_run(int32_t *hist, BucketCtl *ctl) {
+  __shared__ int32_t wtot[kMBins / 64];
+  const int r = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int cnt = hist[r * kMBins + tid];
+  int inc = cnt;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(inc, off);
+    if (lane >= off) inc += u;
+  }
+  if (lane == 63) wtot[w] = inc;
+  __syncthreads();
+  int base = 0;
+  for (int i = 0; i < w; ++i) base += wtot[i];
+  hist[r * kMBins + tid] = base + inc - cnt;
+  if (tid == kMBins - 1) ctl->run_start[r] = base + inc;
+}
+
+__global__ __launch_bounds__(kRuns) void k_scan_runs(BucketCtl *ctl) {
   __shared__ int32_t wsum[2][kRuns / 64];
   const int tid = threadIdx.x;
-  int4 *h4 = reinterpret_cast<int4 *>(hist) + tid * 32;
-  int sum = 0;
-#pragma unroll
-  for (int i = 0; i < 32; ++i) {
-    const int4 v = h4[i];
-    sum += v.x + v.y + v.z + v.w;
-  }
-  part[tid] = sum;
-  __syncthreads();
-  // one thread per run: its aligned size and group count, scanned over the
-  // runs (wave scans, then the wave totals)
-  int al = 0, ng = 0, inc_al = 0, inc_ng = 0;
-  if (tid < kRuns) {
-    const int cnt = part[2 * tid] + part[2 * tid + 1];
-    const int gs = 128 / kClassL[tid / kNssBins];
-    ng = (cnt + gs - 1) / gs;
-    al = (cnt + kRunAlign - 1) / kRunAlign * kRunAlign;
-    inc_al = al;
-    inc_ng = ng;
-    for (int off = 1; off < 64; off <<= 1) {
-      const int ua = __shfl_up(inc_al, off), un = __shfl_up(inc_ng, off);
-      if ((tid & 63) >= off) {
-        inc_al += ua;
-        inc_ng += un;
-      }
-    }
-    if ((tid & 63) == 63) {
-      wsum[0][tid >> 6] = inc_al;
-      wsum[1][tid >> 6] = inc_ng;
+  // the run's aligned size and group count, scanned over the runs (wave
+  // scans, then the wave totals)
+  const int cnt = ctl->run_start[tid];
+  const int gs = 128 / kClassL[tid / kNssBins];
+  const int ng = (cnt + gs - 1) / gs;
+  const int al = (cnt + kRunAlign - 1) / kRunAlign * kRunAlign;
+  int inc_al = al, inc_ng = ng;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int ua = __shfl_up(inc_al, off), un = __shfl_up(inc_ng, off);
+    if ((tid & 63) >= off) {
+      inc_al += ua;
+      inc_ng += un;
     }
   }
-  __syncthreads();
-  if (tid < kRuns) {
-    int pa = 0, pn = 0;
-    for (int w = 0; w < (tid >> 6); ++w) {
-      pa += wsum[0][w];
-      pn += wsum[1][w];
-    }
-    const int start = pa + inc_al - al, gbase = pn + inc_ng - ng;
-    rstart[tid] = start;
-    ctl->run_start[tid] = start;
-    ctl->run_group_base[tid] = gbase;
-    if (tid == kRuns - 1) {
-      ctl->run_group_base[kRuns] = gbase + ng;
-      ctl->ngroups = gbase + ng;
-    }
+  if ((tid & 63) == 63) {
+    wsum[0][tid >> 6] = inc_al;
+    wsum[1][tid >> 6] = inc_ng;
   }
   __syncthreads();
-  int pos = rstart[tid >> 1] + ((tid & 1) ? part[tid - 1] : 0);
-#pragma unroll
-  for (int i = 0; i < 32; ++i) {
-    const int4 v = h4[i];
-    int4 o;
-    o.x = pos;
-    o.y = pos + v.x;
-    o.z = o.y + v.y;
-    o.w = o.z + v.z;
-    pos = o.w + v.w;
-    h4[i] = o;
+  int pa = 0, pn = 0;
+  for (int w = 0; w < (tid >> 6); ++w) {
+    pa += wsum[0][w];
+    pn += wsum[1][w];
+  }
+  const int start = pa + inc_al - al, gbase = pn + inc_ng - ng;
+  ctl->run_start[tid] = start;
+  ctl->run_group_base[tid] = gbase;
+  if (tid == kRuns - 1) {
+    ctl->run_group_base[kRuns] = gbase + ng;
+    ctl->ngroups = gbase + ng;
   }
 }
 
 __global__ void k_scatter(int32_t n, const int32_t *bin_of, const int32_t *rank,
-                          const int32_t *offsets, int32_t *sorted) {
+                          const int32_t *offsets, const BucketCtl *ctl, int32_t *sorted) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int b = bin_of[p];
   if (b < 0) return;
-  sorted[offsets[b] + rank[p]] = p;
+  sorted[ctl->run_start[b / kMBins] + offsets[b] + rank[p]] = p;
 }
 
 __device__ __forceinline__ int cost_bin(int64_t cost) {
@@ -1590,9 +1576,10 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
                        st, n, B.plain == 2 ? 1 : 0, B.sorted, B.a_len, B.b_len, B.ctl, B.groups);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(kScanThreads), 0, st, B.hist, B.ctl);
+  hipLaunchKernelGGL(k_scan_run, dim3(kRuns), dim3(kMBins), 0, st, B.hist, B.ctl);
+  hipLaunchKernelGGL(k_scan_runs, dim3(1), dim3(kRuns), 0, st, B.ctl);
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.rank, B.hist,
-                     B.sorted);
+                     B.ctl, B.sorted);
   const int gb = std::min(nblocks(B.max_groups, kGroupThreads / 64), kGroupBlocks);
   hipLaunchKernelGGL(k_groups, dim3(gb), dim3(kGroupThreads), 0, st, B.max_groups, B.sorted,
                      B.a_len, B.b_len, B.ctl, B.gtmp, B.gbin, B.grank, B.costhist);
