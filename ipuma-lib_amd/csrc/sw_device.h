@@ -43,11 +43,18 @@ constexpr int32_t kSat16 = 32767;
 
 // Group descriptor: one wave's work in the packed kernel.
 struct GroupDesc {
-  int32_t cls;   // class index into kClassL
-  int32_t mpad;  // columns (|A|) padded for the group
-  int32_t nss;   // super-strips (rows / (L * kRows), rounded up)
-  int32_t base;  // first slot in the sorted pair array
+  int32_t cls;        // class index into kClassL
+  int32_t mpad;       // columns (|A|) padded for the group
+  int32_t nss;        // super-strips (rows / (L * kRows), rounded up)
+  int32_t base;       // first slot in the sorted pair array
+  int32_t last_rows;  // rows per lane the last super-strip needs (1..kRows)
 };
+
+// Rows per lane the packed kernel runs the last super-strip of a group with:
+// the smallest of 8, 12 and kRows that covers `need`.
+__host__ __device__ inline int last_strip_rows(int need) {
+  return need <= 8 ? 8 : need <= 12 ? 12 : kRows;
+}
 
 // Control block of the device bucketing (one per batch, zeroed before use).
 struct BucketCtl {

@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "sw_device.h"
 
@@ -127,7 +128,6 @@ template <int MODE, int KP, bool PAIRKEY>
 __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__restrict__ tab,
                                           const GroupDesc gd, uint32_t *__restrict__ wscr,
                                           int lane, int lg_rt, const SimdShare &share) {
-  constexpr int R = kRows;
   constexpr uint32_t PADC = KP - 1;
   constexpr uint32_t COLSTRIDE = KP * KP * 4;  // bytes of one A-code row of the LDS table
   // pipeline depth L = 2^lg: fixed for the solo and whole-wave modes
@@ -137,7 +137,6 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   // |A| <= 16384 (the wave scratch holds mpad * 64/L <= 65536 words); the
   // whole-wave pipeline takes them up to mpad = kPairKeyMaxM and keeps one
   // key per row with 16-bit columns beyond (|A| up to 65535).
-  constexpr int KN = PAIRKEY ? R / 2 : R;
   const int L = 1 << lg;
   const int NG = 64 >> lg;  // pipelines per wave
   const int k = lane & (L - 1);
@@ -169,11 +168,16 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
 
   int bh_lo = 0, bi_lo = 0, bj_lo = 0, bh_hi = 0, bi_hi = 0, bj_hi = 0;
 
-  for (int ss = 0; ss < nss; ++ss) {
+  // One super-strip with R rows per lane: kRows, or for the last super-strip
+  // of a group the 8 or 12 that cover its rows (lane k then owns rows
+  // ss * L * kRows + k * R .. + R).
+  auto strip = [&](auto rows_per_lane, const int ss) {
+    constexpr int R = decltype(rows_per_lane)::value;
+    constexpr int KN = PAIRKEY ? R / 2 : R;
     // The boundary row written by lane L-1 during the previous super-strip is
     // read back by lane 0: complete those stores and drop stale L1 lines.
     if (ss > 0) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    const int row0 = ((ss << lg) + k) * R;
+    const int row0 = (ss << lg) * kRows + k * R;
     uint32_t rp[R / 2];  // row-pair table offsets: low half low comparison, high half high comparison
     {
       uint32_t wl[R / 4], wh[R / 4];
@@ -330,6 +334,18 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
           bj_hi = 0xFFFF - (int)(KH[q] & 0xFFFFu);
         }
       }
+    }
+  };
+  using Full = std::integral_constant<int, kRows>;
+  for (int ss = 0; ss + 1 < nss; ++ss) strip(Full{}, ss);
+  if (nss > 0) {
+    if constexpr (PAIRKEY) {
+      const int rl = last_strip_rows(gd.last_rows);
+      if (rl == 8) strip(std::integral_constant<int, 8>{}, nss - 1);
+      else if (rl == 12) strip(std::integral_constant<int, 12>{}, nss - 1);
+      else strip(Full{}, nss - 1);
+    } else {
+      strip(Full{}, nss - 1);
     }
   }
 
@@ -837,7 +853,8 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
     const int lg = cls < kNumClasses - 1 ? cls : 6;
     const bool ok = cls >= 0 && cls < kNumClasses && gd.base >= 0 && gd.mpad >= 0 &&
                     gd.mpad <= kMaxSeqLen && ((int64_t)gd.mpad << (6 - lg)) <= 65536 &&
-                    gd.nss >= 0 && ((int64_t)(gd.nss - 1) << lg) * kRows < kMaxSeqLen;
+                    gd.nss >= 0 && ((int64_t)(gd.nss - 1) << lg) * kRows < kMaxSeqLen &&
+                    gd.last_rows >= 1 && gd.last_rows <= kRows;
     if (!ok) {
       atomicOr(&A.ctl->error, 4);  // every lane: no lane-conditional branch here either
     } else if (cls == 0) {
@@ -1247,8 +1264,13 @@ __global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, co
       d.mpad = mpad;
       d.nss = (nbmax + L * kRows - 1) / (L * kRows);
       d.base = base;
+      // the last super-strip's rows per lane (its lanes cover the rest of B)
+      d.last_rows = d.nss > 0 ? (nbmax - (d.nss - 1) * L * kRows + L - 1) / L : kRows;
       gtmp[gi] = d;
-      pad += (unsigned long long)128 * d.nss * kRows * (unsigned long long)mpad;
+      const bool band_g = c == kNumClasses - 1 && d.nss >= 2;  // band strips run kRows rows
+      const int rows_last = band_g || d.nss == 0 ? kRows : last_strip_rows(d.last_rows);
+      pad += (unsigned long long)128 * (unsigned long long)mpad *
+             (unsigned long long)(d.nss > 0 ? (d.nss - 1) * kRows + rows_last : 0);
       // longest groups first over the whole batch (a wave's time is about its
       // super-strips times its steps per super-strip)
       // band groups (whole-wave pipelines over two or more super-strips,
@@ -1408,6 +1430,7 @@ __global__ __launch_bounds__(256) void k_plain_groups(int32_t n, int32_t per_gro
   d.mpad = mpad;
   d.nss = (nbmax + L * kRows - 1) / (L * kRows);
   d.base = g * gs;
+  d.last_rows = kRows;  // the padded baselines keep full super-strips
   groups[g] = d;
   atomicAdd(&ctl->padded_cells,
             (unsigned long long)128 * d.nss * kRows * (unsigned long long)d.mpad);

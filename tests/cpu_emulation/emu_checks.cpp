@@ -49,8 +49,8 @@ int main() {
     std::vector<int32_t> sorted(256, -1);
     sorted[0] = 0;
     GroupDesc gd[2];
-    gd[0] = {0, 8, 0x01020304, 0};  // nss from stray residue bytes
-    gd[1] = {0, 8, 1, 0};           // the real group
+    gd[0] = {0, 8, 0x01020304, 0, 16};  // nss from stray residue bytes
+    gd[1] = {0, 8, 1, 0, 8};            // the real group (8 rows in its one super-strip)
     BucketCtl ctl;
     std::memset(&ctl, 0, sizeof(ctl));
     ctl.ngroups = 2;
