@@ -58,7 +58,10 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
     (void)first;
     return (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x138, 0xf, 0xf, false);
   } else {
-    const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x111, 0xf, 0xf, false);
+    // DPP row_shr:1 inside rows of 16 lanes; a lane without a source (a row
+    // start) reads 0 (bound_ctrl), and every first lane of a pipeline (row
+    // starts included) selects `own`
+    const uint32_t r = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x111, 0xf, 0xf, true);
     return first ? own : r;
   }
 }
@@ -234,8 +237,9 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
       for (int u = 0; u < 4; ++u) {
         const int t = t0 + u;
         const uint2 cb = nb;
+        // byte offsets into the wave scratch fit 32 bits (it holds 512 KiB)
         if (first && load_bnd && t + 1 < mpad)
-          nb = *(const uint2 *)(wscr + ((size_t)(t + 1) * NG + g) * 2);
+          nb = *(const uint2 *)((const char *)wscr + (uint32_t)(((t + 1) * NG + g) * 8));
         else
           nb = make_uint2(0u, 0u);
         const uint32_t ac_lo = (t < m_lo) ? min((wa_lo >> (8 * u)) & 0xffu, PADC) : PADC;
@@ -306,7 +310,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         out_h = H[R - 1];
         out_f = f;
         if (last && store_bnd && j >= 0 && j < mpad)
-          *(uint2 *)(wscr + ((size_t)j * NG + g) * 2) = make_uint2(out_h, out_f);
+          *(uint2 *)((char *)wscr + (uint32_t)((j * NG + g) * 8)) = make_uint2(out_h, out_f);
       }
     }
     // rows (or row pairs) in order, strict >: the first maximum in B order

@@ -135,6 +135,7 @@ void launch(const char *name, K kernel, dim3 grid, dim3 block, Args... args) {
 #define gridDim (emu::ctx.gdim)
 #define __syncthreads() (emu::ctx.block->arrive_and_wait())
 #define __builtin_amdgcn_update_dpp(o, s, c, r, b, bc) emu::update_dpp((o), (s), (c), (r), (b), (bc))
+#define __builtin_amdgcn_mov_dpp(s, c, r, b, bc) emu::update_dpp(0, (s), (c), (r), (b), (bc))
 #define __builtin_amdgcn_perm(a, b, s) emu::perm((a), (b), (s))
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_readfirstlane(v) emu::exchange_get((v), 0)
