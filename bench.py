@@ -49,6 +49,10 @@ def parse():
     ap.add_argument("--with-starts", action="store_true", help="time the reverse pass too")
     ap.add_argument("--rounds", type=int, default=0,
                     help="groups per resident wave for the bucketing (0 = engine default)")
+    ap.add_argument("--autotune", choices=["on", "off"], default="on",
+                    help="time a small grid of bucketing shapes (rounds x step cap scale) on "
+                         "the batch before the timed region and keep the fastest (off with "
+                         "--rounds)")
     ap.add_argument("--serve-rates", default="200",
                     help="comma-separated request rates (req/s) of the serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
@@ -266,6 +270,16 @@ def main():
         eng.compare_device(codes, codes.numel(), a_off, a_len, b_off, b_len, scores, ar, br,
                            stream=sptr)
 
+    tuned = None
+    if args.autotune == "on" and not args.rounds:
+        # bucketing shape chosen on this batch, outside the timed region
+        from ipuma_amd import autotune as AT
+        key = AT.signature(batch["a_len"], batch["b_len"], batch["sw"].get("similarity", "dna"))
+        tuned = AT.tune(eng, step, key=key, sync=torch.cuda.synchronize)
+        tab = ", ".join("%dx%.2f: %.2f ms" % (r["rounds"], r["scale"], r["ms"])
+                        for r in tuned["table"])
+        log(f"autotune {key}: rounds {tuned['rounds']} scale {tuned['scale']} ({tab})")
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -359,7 +373,9 @@ def main():
         "data": data,
         "config": {"workload": workload,
                    "pairs": n, "cells": cells,
-                   "starts": bool(args.with_starts), "work_rounds": args.rounds or "default"},
+                   "starts": bool(args.with_starts), "work_rounds": args.rounds or "default",
+                   "autotune": ({"rounds": tuned["rounds"], "step_cap_scale": tuned["scale"],
+                                 "candidates": len(tuned["table"])} if tuned else None)},
         "bucketed_vs_plain": (round(gcups / plain["gcups"], 2) if plain else None),
         "plain": plain,
         "parity_2000_pairs": parity,

@@ -245,6 +245,12 @@ int ipuma_set_lanes_per_pair(ipuma_engine *e, int32_t lanes);
  * price of deeper pipelines (more row padding) per comparison.  No reference
  * counterpart (the IPU's work split is fixed by its tile mapping). */
 int ipuma_set_work_rounds(ipuma_engine *e, int32_t rounds);
+/* Scale of the bucketing's cap on a group's steps (0.25..4, default 1): above
+ * 1, fewer and longer groups (shallower pipelines, less row padding); below 1,
+ * more and shorter groups (a finer longest-first queue).  The knob the
+ * autotuner (ipuma_amd.autotune) searches together with the rounds.  No
+ * reference counterpart (the IPU's work split is fixed by its tile mapping). */
+int ipuma_set_step_cap_scale(ipuma_engine *e, float scale);
 
 /* Encode characters to residue codes on the device with the engine's
  * encoding (swatlib Encoding, src/swatlib/encoding.cpp:40-62). */

@@ -173,6 +173,9 @@ struct BucketArgs {
 #include <hip/hip_runtime.h>
 namespace ipuma {
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+// Up to four ranges filled by one launch (a range with count 0 is skipped).
+hipError_t launch_fill_ranges(int32_t *const p[4], const int64_t n[4], const int32_t v[4],
+                              hipStream_t st);
 hipError_t launch_encode(const uint8_t *chars, int64_t len, uint8_t *codes, const uint8_t *table,
                          hipStream_t st);
 hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st);

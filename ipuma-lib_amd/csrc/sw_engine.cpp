@@ -125,6 +125,7 @@ struct ipuma_engine {
   int generic_blocks = 512;
   int64_t target_slots = 0;
   int rounds = 1;  // groups per resident wave the bucketing aims for
+  int cap_milli = 1000;  // step cap scale, in thousandths
   DevBuf scratch, band_scratch, work;
   ChainBufs fwd, rev;
   DevBuf rcodes, rlen, rres;
@@ -160,6 +161,13 @@ struct ipuma_engine {
   std::thread pj_thread;
   bool pj_stop = false;
 };
+
+// Resident half-lanes x rounds, divided by the step cap scale: the bucketing's
+// cap on a group's steps is cells / (kRows * target_slots).
+static void update_target_slots(ipuma_engine *e) {
+  const int64_t base = (int64_t)e->fast_blocks * (kFastThreads / 64) * 128 * e->rounds;
+  e->target_slots = std::max<int64_t>(1, base * 1000 / std::max(1, e->cap_milli));
+}
 
 namespace {
 
@@ -197,7 +205,7 @@ int init_engine(ipuma_engine *e) {
   e->forward_only = e->algo.forward_only ? 1 : 0;
   e->fast_blocks = e->num_cu * fast_blocks_per_cu(e->kp);
   e->generic_blocks = std::max(8, 2 * e->num_cu);
-  e->target_slots = (int64_t)e->fast_blocks * (kFastThreads / 64) * 128 * e->rounds;
+  update_target_slots(e);
   return IPUMA_OK;
 }
 
@@ -224,8 +232,8 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   if (rc) return rc;
   const bool packed = e->packed_ok && e->precision == IPUMA_PREC_AUTO;
   BucketCtl *ctl = c.ctl.as<BucketCtl>();
-  HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(ctl), sizeof(BucketCtl) / 4, 0, st));
   if (!packed) {
+    HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(ctl), sizeof(BucketCtl) / 4, 0, st));
     // every comparison through the exact int32 kernel
     if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
     GenericArgs G{};
@@ -241,9 +249,14 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   if ((rc = e->band_scratch.ensure((size_t)e->fast_blocks * kBandSlots * kWaveScratchWords * 4)))
     return rc;
   if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
-  HIP_TRY(launch_fill_i32(c.hist.as<int32_t>(), kNumBins, 0, st));
-  HIP_TRY(launch_fill_i32(c.costhist.as<int32_t>(), kCostBins, 0, st));
-  HIP_TRY(launch_fill_i32(c.sorted.as<int32_t>(), (int64_t)n + kSortedPad, -1, st));
+  {  // control block, bin and cost histograms cleared, sorted slots empty
+    int32_t *const fp[4] = {reinterpret_cast<int32_t *>(ctl), c.hist.as<int32_t>(),
+                            c.costhist.as<int32_t>(), c.sorted.as<int32_t>()};
+    const int64_t fn[4] = {(int64_t)(sizeof(BucketCtl) / 4), kNumBins, kCostBins,
+                           (int64_t)n + kSortedPad};
+    const int32_t fv[4] = {0, 0, 0, -1};
+    HIP_TRY(launch_fill_ranges(fp, fn, fv, st));
+  }
   BucketArgs B{};
   B.n = n; B.a_len = a_len; B.b_len = b_len; B.target_slots = e->target_slots;
   B.force_class = e->force_class; B.plain = e->batching == IPUMA_BATCH_PLAIN ? 1 : e->batching == IPUMA_BATCH_PLAIN_TILE ? 2 : 0;
@@ -814,7 +827,16 @@ int ipuma_set_work_rounds(ipuma_engine *e, int32_t rounds) {
   std::lock_guard<std::mutex> lock_(e->mu);
   if (rounds < 1 || rounds > 64) return fail(IPUMA_EINVAL, "rounds must be in 1..64");
   e->rounds = rounds;
-  e->target_slots = (int64_t)e->fast_blocks * (kFastThreads / 64) * 128 * e->rounds;
+  update_target_slots(e);
+  return IPUMA_OK;
+}
+
+int ipuma_set_step_cap_scale(ipuma_engine *e, float scale) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
+  if (!(scale >= 0.25f && scale <= 4.0f)) return fail(IPUMA_EINVAL, "scale must be in 0.25..4");
+  e->cap_milli = (int)(scale * 1000.0f + 0.5f);
+  update_target_slots(e);
   return IPUMA_OK;
 }
 

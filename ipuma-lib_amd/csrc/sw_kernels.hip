@@ -958,6 +958,24 @@ __global__ __launch_bounds__(64) void k_generic32(GenericArgs G) {
 // ---------------------------------------------------------------------------
 // Small utility kernels.
 // ---------------------------------------------------------------------------
+// The bucketing's buffers cleared in one launch: up to four (pointer, count,
+// value) ranges over one grid-stride index.
+struct FillRanges {
+  int32_t *p[4];
+  int64_t n[4];
+  int32_t v[4];
+};
+__global__ void k_fill_ranges(FillRanges f) {
+  const int64_t total = f.n[0] + f.n[1] + f.n[2] + f.n[3];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = i;
+    int s = 0;
+    while (s < 3 && j >= f.n[s]) j -= f.n[s++];
+    f.p[s][j] = f.v[s];
+  }
+}
+
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -1141,27 +1159,39 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
   if (tid < kNumClasses && s_count[tid]) atomicAdd(&ctl->class_count[tid], s_count[tid]);
 }
 
-// Scan of the bins in two kernels.  k_scan_run: one block per run (class,
-// super-strip bin) of kMBins consecutive bins turns each bin's count into
-// its offset inside the run and leaves the run's total in run_start.
-// k_scan_runs: one thread per run gives every run its start (a multiple of
-// kRunAlign slots) and its first group.  k_scatter adds the two.
+// Scan of the bins in two kernels.  k_scan_run: one wave per run (class,
+// super-strip bin) of kMBins consecutive bins, four bins per lane, turns each
+// bin's count into its offset inside the run and leaves the run's total in
+// run_start.  k_scan_runs: one thread per run gives every run its start (a
+// multiple of kRunAlign slots) and its first group.  k_scatter adds the two.
 static_assert(kMBins == 256 && kRuns % 64 == 0 && kRuns <= 1024, "scan layout");
-__global__ __launch_bounds__(kMBins) void k_scan_run(int32_t *hist, BucketCtl *ctl) {
-  __shared__ int32_t wtot[kMBins / 64];
-  const int r = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int cnt = hist[r * kMBins + tid];
-  int inc = cnt;
-  for (int off = 1; off < 64; off <<= 1) {
-    const int u = __shfl_up(inc, off);
-    if (lane >= off) inc += u;
+constexpr int kScanRunBlocks = 16;  // x 4 waves: each wave scans every 64th run
+__global__ __launch_bounds__(256) void k_scan_run(int32_t *hist, BucketCtl *ctl) {
+  const int lane = threadIdx.x & 63;
+  const int nw = (int)gridDim.x * (int)(blockDim.x >> 6);
+  for (int r = (int)blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6); r < kRuns;
+       r += nw) {
+    int4 *h4 = reinterpret_cast<int4 *>(hist + r * kMBins) + lane;
+    const int4 c = *h4;
+    const int tot = c.x + c.y + c.z + c.w;
+    if (__ballot(tot != 0) == 0) {  // an empty run: its offsets are already 0
+      if (lane == 63) ctl->run_start[r] = 0;
+      continue;
+    }
+    int inc = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int u = __shfl_up(inc, off);
+      if (lane >= off) inc += u;
+    }
+    const int base = inc - tot;
+    int4 o;
+    o.x = base;
+    o.y = base + c.x;
+    o.z = o.y + c.y;
+    o.w = o.z + c.z;
+    *h4 = o;
+    if (lane == 63) ctl->run_start[r] = inc;
   }
-  if (lane == 63) wtot[w] = inc;
-  __syncthreads();
-  int base = 0;
-  for (int i = 0; i < w; ++i) base += wtot[i];
-  hist[r * kMBins + tid] = base + inc - cnt;
-  if (tid == kMBins - 1) ctl->run_start[r] = base + inc;
 }
 
 __global__ __launch_bounds__(kRuns) void k_scan_runs(BucketCtl *ctl) {
@@ -1575,6 +1605,22 @@ __global__ __launch_bounds__(256) void k_wire_gather(int64_t nseq, const uint8_t
 // ---------------------------------------------------------------------------
 static inline int nblocks(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
+hipError_t launch_fill_ranges(int32_t *const p[4], const int64_t n[4], const int32_t v[4],
+                              hipStream_t st) {
+  FillRanges f;
+  int64_t total = 0;
+  for (int i = 0; i < 4; ++i) {
+    f.p[i] = p[i];
+    f.n[i] = std::max<int64_t>(n[i], 0);
+    f.v[i] = v[i];
+    total += f.n[i];
+  }
+  if (total <= 0) return hipSuccess;
+  const int blocks = (int)std::min<int64_t>(nblocks(total, 256), 128);
+  hipLaunchKernelGGL(k_fill_ranges, dim3(blocks), dim3(256), 0, st, f);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const int blocks = (int)std::min<int64_t>(nblocks(n, 256), 4096);
@@ -1603,7 +1649,7 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
                        st, n, B.plain == 2 ? 1 : 0, B.sorted, B.a_len, B.b_len, B.ctl, B.groups);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_scan_run, dim3(kRuns), dim3(kMBins), 0, st, B.hist, B.ctl);
+  hipLaunchKernelGGL(k_scan_run, dim3(kScanRunBlocks), dim3(256), 0, st, B.hist, B.ctl);
   hipLaunchKernelGGL(k_scan_runs, dim3(1), dim3(kRuns), 0, st, B.ctl);
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.rank, B.hist,
                      B.ctl, B.sorted);

@@ -149,6 +149,7 @@ def lib():
     L.ipuma_set_forward_only.argtypes = [_P, _i32]
     L.ipuma_set_lanes_per_pair.argtypes = [_P, _i32]
     L.ipuma_set_work_rounds.argtypes = [_P, _i32]
+    L.ipuma_set_step_cap_scale.argtypes = [_P, ctypes.c_float]
     L.ipuma_set_timing.argtypes = [_P, _i32]
     L.ipuma_encode_device.argtypes = [_P, _P, _i64, _P, _P]
     L.ipuma_last_batch_stats.argtypes = [_P, ctypes.POINTER(BatchStats)]
@@ -333,6 +334,10 @@ class SWAlgorithm:
     def set_work_rounds(self, rounds: int):
         """Groups per resident wave the length bucketing aims for (1..64)."""
         _check(lib().ipuma_set_work_rounds(self._h, int(rounds)))
+
+    def set_step_cap_scale(self, scale: float):
+        """Scale of the bucketing's step cap per group (0.25..4, default 1)."""
+        _check(lib().ipuma_set_step_cap_scale(self._h, float(scale)))
 
     # --- reference surface --------------------------------------------------
     def _store(self, s, a, b):

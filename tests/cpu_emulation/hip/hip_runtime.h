@@ -104,28 +104,33 @@ void launch(const char *name, K kernel, dim3 grid, dim3 block, Args... args) {
   const unsigned nthreads = block.x;
   if (std::getenv("EMU_TRACE")) std::fprintf(stderr, "emu launch %s grid %u block %u\n", name, grid.x, block.x);
   if (nthreads % 64) { std::fprintf(stderr, "emu: block size %u not a wave multiple\n", nthreads); std::abort(); }
-  for (unsigned b = 0; b < grid.x; ++b) {
-    std::barrier<> blockbar(nthreads);
-    std::vector<std::unique_ptr<Wave>> waves;
-    for (unsigned w = 0; w < nthreads / 64; ++w) waves.emplace_back(new Wave());
-    std::vector<std::thread> th;
-    th.reserve(nthreads);
-    for (unsigned t = 0; t < nthreads; ++t)
-      th.emplace_back([&, t]() {
-        ctx.tid = dim3(t);
+  // One thread per lane for the whole grid: the blocks run one after another
+  // on the same threads, separated by a barrier outside the kernel.
+  std::barrier<> blockbar(nthreads), endbar(nthreads);
+  std::vector<std::unique_ptr<Wave>> waves;
+  for (unsigned w = 0; w < nthreads / 64; ++w) waves.emplace_back(new Wave());
+  std::vector<std::thread> th;
+  th.reserve(nthreads);
+  for (unsigned t = 0; t < nthreads; ++t)
+    th.emplace_back([&, t]() {
+      ctx.tid = dim3(t);
+      ctx.bdim = block;
+      ctx.gdim = grid;
+      ctx.lane = (int)(t & 63);
+      ctx.wave = waves[t / 64].get();
+      ctx.block = &blockbar;
+      for (unsigned b = 0; b < grid.x; ++b) {
         ctx.bid = dim3(b);
-        ctx.bdim = block;
-        ctx.gdim = grid;
-        ctx.lane = (int)(t & 63);
-        ctx.wave = waves[t / 64].get();
-        ctx.block = &blockbar;
         kernel(args...);
         // a thread that returns keeps arriving at its wave / block barriers
         // nowhere: kernels here only return wave-uniformly after their last
-        // exchange, or never exchange at all.
-      });
-    for (auto &x : th) x.join();
-  }
+        // exchange, or never exchange at all, so the barriers' phases agree
+        // when the next block starts.
+        endbar.arrive_and_wait();
+      }
+    });
+  for (auto &x : th) x.join();
+  if (std::getenv("EMU_TRACE")) std::fprintf(stderr, "emu done %s\n", name);
 }
 }  // namespace emu
 
