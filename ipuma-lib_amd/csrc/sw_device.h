@@ -55,6 +55,11 @@ struct GroupDesc {
 __host__ __device__ inline int last_strip_rows(int need) {
   return need <= 8 ? 8 : need <= 12 ? 12 : kRows;
 }
+// Cost of one step with `rows` rows per lane, in 32nds of a kRows step (the
+// step bodies take about 135, 187 and 240 VALU instructions).
+__host__ __device__ inline int step_weight32(int rows) {
+  return rows <= 8 ? 18 : rows <= 12 ? 25 : 32;
+}
 
 // Control block of the device bucketing (one per batch, zeroed before use).
 struct BucketCtl {

@@ -174,6 +174,8 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   // One super-strip with R rows per lane: kRows, or for the last super-strip
   // of a group the 8 or 12 that cover its rows (lane k then owns rows
   // ss * L * kRows + k * R .. + R).
+  // weight of the group's last super-strip (its rows per lane)
+  const int wlast = step_weight32(PAIRKEY ? last_strip_rows(gd.last_rows) : kRows);
   auto strip = [&](auto rows_per_lane, const int ss) {
     constexpr int R = decltype(rows_per_lane)::value;
     constexpr int KN = PAIRKEY ? R / 2 : R;
@@ -226,7 +228,9 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     }
     const int T = mpad + L - 1;
     for (int t0 = 0; t0 < T; t0 += 4) {
-      if ((t0 & 7) == 0) share_simd(share, (nss - ss) * T - t0, lane);
+      if ((t0 & 7) == 0)  // work left, in 32nds of a kRows step
+        share_simd(share, ((nss - 1 - ss) * 32 - 32 + wlast) * T + (T - t0) * step_weight32(R) +
+                              (ss + 1 == nss ? (32 - wlast) * T : 0), lane);
       const uint32_t wa_lo = na_lo, wa_hi = na_hi;
       if (first) {
         const int tn = t0 + 4;
@@ -1310,8 +1314,10 @@ __global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, co
       // band groups (whole-wave pipelines over two or more super-strips,
       // shared by a block) come before all others
       const bool band = c == kNumClasses - 1 && d.nss >= 2;
+      const int64_t strips32 =
+          d.nss > 0 ? (int64_t)(d.nss - 1) * 32 + step_weight32(rows_last) : 0;
       const int cb = (band ? 0 : kCostBinsPerClass) + kCostBinsPerClass - 1 -
-                     cost_bin((int64_t)d.nss * (mpad + L - 1));
+                     cost_bin(strips32 * (mpad + L - 1) / 32);
       gbin[gi] = cb;
       grank[gi] = atomicAdd(&s_cost[cb], 1);
       if (band) atomicAdd(&s_nband, 1);
