@@ -210,7 +210,7 @@ def run_search(args):
                    "planted_homologs_found": f"{found}/{len(planted)}"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s (int16)", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                     "traffic": None, "kernel": "k_fast<26>", "kernel_ms": round(fast_avg, 3),
+                     "traffic": None, "kernel": "k_fast<26, false>", "kernel_ms": round(fast_avg, 3),
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,
     }
@@ -384,7 +384,7 @@ def main():
                      "traffic": tr["traffic_bytes"] if tr else None,
                      "traffic_source": tr["source"] if tr else None,
                      "hbm_gbs": (round(tr["traffic_bytes"] / (fast_avg * 1e6), 1) if tr else None),
-                     "kernel": f"k_fast<{26 if args.config == 'protein' else 18}>",
+                     "kernel": f"k_fast<{26 if args.config == 'protein' else 18}, false>",
                      "kernel_ms": round(fast_avg, 3),
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,

@@ -100,6 +100,12 @@ struct FastArgs {
   int32_t *ovf_list;           // pairs that saturated int16
   int32_t *ovf_count;
   int32_t max_groups;          // bound of the group count
+  // Start pass only: the forward score of each pair.  The maximum over the
+  // reversed prefixes equals it, so a group stops after the first
+  // super-strip by which every pair has reached it (the first maximum in
+  // B-major order lies in that super-strip or an earlier one); nullptr: run
+  // every super-strip.
+  const int32_t *target;
   uint32_t gap_o;              // -(gapInit + gapExtend), packed twice
   uint32_t gap_x;              // -gapExtend, packed twice
 };

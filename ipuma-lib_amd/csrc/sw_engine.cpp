@@ -227,7 +227,7 @@ int ensure_chain(ChainBufs &c, int32_t n) {
 int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
               const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
               const int32_t *b_len, int32_t *scores, int32_t *a_range, int32_t *b_range,
-              hipStream_t st) {
+              hipStream_t st, const int32_t *target) {
   int rc = ensure_chain(c, n);
   if (rc) return rc;
   const bool packed = e->packed_ok && e->precision == IPUMA_PREC_AUTO;
@@ -273,6 +273,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   A.scores = scores; A.a_range = a_range; A.b_range = b_range;
   A.ovf_list = c.ovf.as<int32_t>(); A.ovf_count = &ctl->ovf_count;
   A.max_groups = n + kRuns;
+  A.target = target;
   const uint32_t o = (uint32_t)(-(e->sw.gap_init + e->sw.gap_extend)) & 0xFFFFu;
   const uint32_t x = (uint32_t)(-e->sw.gap_extend) & 0xFFFFu;
   A.gap_o = o | (o << 16);
@@ -329,7 +330,8 @@ int compare_device_chain(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
                          int32_t *b_range, hipStream_t st) {
   if (e->timing) HIP_TRY(hipEventRecord(e->ev[0], st));
   e->ev_recorded = false;
-  int rc = run_chain(e, e->fwd, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st);
+  int rc = run_chain(e, e->fwd, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st,
+                     nullptr);
   if (rc) return rc;
   e->last_ctl = e->fwd.ctl.as<BucketCtl>();
   e->last_n = n;
@@ -349,8 +351,9 @@ int compare_device_chain(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
   HIP_TRY(launch_reverse_prefix(n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range,
                                 e->rcodes.as<uint8_t>(), ra_len, rb_len,
                                 &e->fwd.ctl.as<BucketCtl>()->error, st));
+  // the forward scores are the targets: a group stops once every pair reached its own
   rc = run_chain(e, e->rev, n, e->rcodes.as<uint8_t>(), a_off, ra_len, b_off, rb_len, rs, rar, rbr,
-                 st);
+                 st, scores);
   if (rc) return rc;
   HIP_TRY(launch_finish_starts(n, scores, rar, rbr, a_range, b_range, st));
   if (e->timing) {

@@ -127,7 +127,7 @@ __device__ __forceinline__ void share_simd(const SimdShare &S, int left, int lan
   else __builtin_amdgcn_s_setprio(2);
 }
 
-template <int MODE, int KP, bool PAIRKEY>
+template <int MODE, int KP, bool PAIRKEY, bool STOP>
 __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__restrict__ tab,
                                           const GroupDesc gd, uint32_t *__restrict__ wscr,
                                           int lane, int lg_rt, const SimdShare &share) {
@@ -344,9 +344,28 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
       }
     }
   };
+  // Start pass: stop after the super-strip by which every pair of the wave has
+  // reached its target (a pipeline has it when any of its lanes does).
+  // (compiled into the start pass's kernel only: the check's registers would
+  // push the forward kernel's step loops into spilling)
+  const int32_t *const target = A.target;
   using Full = std::integral_constant<int, kRows>;
-  for (int ss = 0; ss + 1 < nss; ++ss) strip(Full{}, ss);
-  if (nss > 0) {
+  bool reached = false;
+  for (int ss = 0; ss + 1 < nss; ++ss) {
+    strip(Full{}, ss);
+    if constexpr (STOP) {
+      const int tg_lo = p_lo >= 0 ? target[p_lo] : INT32_MIN;
+      const int tg_hi = p_hi >= 0 ? target[p_hi] : INT32_MIN;
+      const uint64_t mlo = __ballot(bh_lo >= tg_lo), mhi = __ballot(bh_hi >= tg_hi);
+      const uint64_t win = L == 64 ? ~0ull : ((1ull << L) - 1ull);
+      const bool ok = !first || ((((mlo >> lane) & win) != 0) && (((mhi >> lane) & win) != 0));
+      if (__ballot(!ok) == 0) {
+        reached = true;
+        break;
+      }
+    }
+  }
+  if (nss > 0 && !reached) {
     if constexpr (PAIRKEY) {
       const int rl = last_strip_rows(gd.last_rows);
       if (rl == 8) strip(std::integral_constant<int, 8>{}, nss - 1);
@@ -812,7 +831,7 @@ __device__ __forceinline__ int next_group(int32_t *queue, int first, int ng, int
 // One persistent kernel for the whole batch: waves take groups from a queue
 // ordered longest first over all pipeline depths, so deep and shallow
 // pipelines run side by side and the tail is short.
-template <int KP>
+template <int KP, bool STOP>
 __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   constexpr int WPB = kFastThreads / 64;
   __shared__ uint32_t tab[KP * KP * KP];
@@ -866,14 +885,14 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
     if (!ok) {
       atomicOr(&A.ctl->error, 4);  // every lane: no lane-conditional branch here either
     } else if (cls == 0) {
-      run_group<kModeSolo, KP, true>(A, tab, gd, wscr, lane, 0, share);
+      run_group<kModeSolo, KP, true, STOP>(A, tab, gd, wscr, lane, 0, share);
     } else if (cls == kNumClasses - 1) {
       if (gd.mpad <= kPairKeyMaxM)
-        run_group<kModeWave, KP, true>(A, tab, gd, wscr, lane, 6, share);
+        run_group<kModeWave, KP, true, STOP>(A, tab, gd, wscr, lane, 6, share);
       else
-        run_group<kModeWave, KP, false>(A, tab, gd, wscr, lane, 6, share);
+        run_group<kModeWave, KP, false, STOP>(A, tab, gd, wscr, lane, 6, share);
     } else {
-      run_group<kModeRow, KP, true>(A, tab, gd, wscr, lane, lg, share);
+      run_group<kModeRow, KP, true, STOP>(A, tab, gd, wscr, lane, lg, share);
     }
   }
   if (lane == 0)
@@ -881,8 +900,10 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   __builtin_amdgcn_s_setprio(0);
 }
 
-template __global__ void k_fast<18>(FastArgs);
-template __global__ void k_fast<26>(FastArgs);
+template __global__ void k_fast<18, false>(FastArgs);
+template __global__ void k_fast<26, false>(FastArgs);
+template __global__ void k_fast<18, true>(FastArgs);
+template __global__ void k_fast<26, true>(FastArgs);
 
 // ---------------------------------------------------------------------------
 // Exact int32 kernel: the vertex loop of codelet_batch_affine.cpp:62-87 with
@@ -1685,19 +1706,32 @@ hipError_t launch_wire_gather(int64_t nseq, const uint8_t *wire, const int64_t *
 }
 
 hipError_t launch_fast(int kp, const FastArgs &A, int blocks, hipStream_t st) {
-  if (kp == 18)
-    hipLaunchKernelGGL(k_fast<18>, dim3(blocks), dim3(kFastThreads), 0, st, A);
-  else
-    hipLaunchKernelGGL(k_fast<26>, dim3(blocks), dim3(kFastThreads), 0, st, A);
+  // the start pass (targets set) runs the kernel with the early stop compiled in
+  if (A.target != nullptr) {
+    if (kp == 18)
+      hipLaunchKernelGGL((k_fast<18, true>), dim3(blocks), dim3(kFastThreads), 0, st, A);
+    else
+      hipLaunchKernelGGL((k_fast<26, true>), dim3(blocks), dim3(kFastThreads), 0, st, A);
+  } else if (kp == 18) {
+    hipLaunchKernelGGL((k_fast<18, false>), dim3(blocks), dim3(kFastThreads), 0, st, A);
+  } else {
+    hipLaunchKernelGGL((k_fast<26, false>), dim3(blocks), dim3(kFastThreads), 0, st, A);
+  }
   return hipGetLastError();
 }
 
 int fast_blocks_per_cu(int kp) {
-  int nb = 0;
+  // resident blocks of both instances (forward, and start pass with its stop)
+  int nb = 0, ns = 0;
   hipError_t e = kp == 18
-      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<18>, kFastThreads, 0)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<26>, kFastThreads, 0);
-  return (e == hipSuccess && nb > 0) ? nb : 1;
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<18, false>, kFastThreads, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<26, false>, kFastThreads, 0);
+  hipError_t e2 = kp == 18
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_fast<18, true>, kFastThreads, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&ns, k_fast<26, true>, kFastThreads, 0);
+  if (e != hipSuccess || e2 != hipSuccess) return 1;
+  const int m = std::min(nb, ns);
+  return m > 0 ? m : 1;
 }
 
 hipError_t launch_generic(const GenericArgs &G, int blocks, hipStream_t st) {
