@@ -220,7 +220,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
-              "(pairs are independent: see DESIGN.md for how the batch would shard)",
+              "(pairs are independent: a batch shards by cumulative cells, one engine per GPU, "
+              "with a gather of the results by pair index)",
               file=sys.stderr)
         return 2
     if args.config == "search":
