@@ -97,6 +97,31 @@ def test_ssw_parity_pairs_emulated(emu_lib):
     eng.close()
 
 
+def test_shape_knobs_emulated(emu_lib):
+    """The autotuner's knobs through the C ABI: ranges enforced, and a batch
+    under the extreme settings still matches the oracle."""
+    import oracle as O
+    I = emu_lib
+    eng = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1,
+                                        forwardOnly=True))
+    for bad in (0.0, 0.2, 4.5, float("nan")):
+        with pytest.raises(I.IpumaError):
+            eng.set_step_cap_scale(bad)
+    with pytest.raises(I.IpumaError):
+        eng.set_work_rounds(0)
+    rng = np.random.default_rng(41)
+    A = ["".join(rng.choice(list("ACGT"), int(rng.integers(20, 200)))) for _ in range(12)]
+    B = ["".join(rng.choice(list("ACGT"), int(rng.integers(20, 300)))) for _ in range(12)]
+    exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1, forward_only=True)
+    for rounds, scale in ((1, 0.25), (4, 4.0)):
+        eng.set_work_rounds(rounds)
+        eng.set_step_cap_scale(scale)
+        for g, e in zip(eng.compare(A, B), exp):
+            assert np.array_equal(np.asarray(g), np.asarray(e)), (rounds, scale)
+    eng.close()
+
+
 def test_request_server_emulated(emu_lib):
     """Three submitting threads, a 2 KiB arena (so admission blocks and the ring
     wraps), results checked against the oracle; then a request larger than the

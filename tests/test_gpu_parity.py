@@ -407,6 +407,14 @@ def test_errors(gpu):
         eng.compare_local(["A" * 11], ["A"])  # |A| > maxAB
     with pytest.raises(I.IpumaError):
         I.SWAlgorithm(I.SWConfig(similarity="simple"), I.IPUAlgoConfig())
+    for bad in (0.0, 0.2, 4.5, float("nan")):
+        with pytest.raises(I.IpumaError):
+            eng.set_step_cap_scale(bad)
+    for bad in (0, 65):
+        with pytest.raises(I.IpumaError):
+            eng.set_work_rounds(bad)
+    eng.set_step_cap_scale(0.25)
+    eng.set_step_cap_scale(4.0)
 
 
 def test_cpp_facade(gpu):
