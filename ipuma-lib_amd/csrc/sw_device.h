@@ -160,6 +160,10 @@ struct BucketArgs {
   const int32_t *b_len;
   int64_t target_slots;   // resident half-lanes x rounds: step cap = cells / (kRows * this)
   int32_t force_class;    // -1: choose per pair
+  // Start pass only: the forward scores.  A pair's super-strip bin then counts
+  // the rows it is expected to need before reaching its score (2 * score + 16,
+  // at most |B|), so pairs that stop early share groups; nullptr otherwise.
+  const int32_t *target;
   int32_t plain;          // 0: bucketed; 1: input order, batch-max padding; 2: input order,
                           // each group padded to its own longest pair
   int32_t max_groups;

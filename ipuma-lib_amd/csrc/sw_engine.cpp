@@ -264,7 +264,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   B.hist = c.hist.as<int32_t>(); B.grank = c.grank.as<int32_t>();
   B.costhist = c.costhist.as<int32_t>(); B.ctl = ctl; B.sorted = c.sorted.as<int32_t>();
   B.gtmp = c.gtmp.as<GroupDesc>(); B.gbin = c.gbin.as<int32_t>(); B.groups = c.groups.as<GroupDesc>();
-  B.scores = scores; B.a_range = a_range; B.b_range = b_range;
+  B.scores = scores; B.a_range = a_range; B.b_range = b_range; B.target = target;
   HIP_TRY(launch_bucketing(B, st));
   FastArgs A{};
   A.codes = codes; A.a_off = a_off; A.a_len = a_len; A.b_off = b_off; A.b_len = b_len;

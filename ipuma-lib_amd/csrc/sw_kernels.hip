@@ -1102,7 +1102,8 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
                                                           int32_t force_class, int32_t *bin_of,
                                                           int32_t *rank, int32_t *hist,
                                                           BucketCtl *ctl, int32_t *scores,
-                                                          int32_t *a_range, int32_t *b_range) {
+                                                          int32_t *a_range, int32_t *b_range,
+                                                          const int32_t *target) {
   __shared__ int s_count[kNumClasses];
   __shared__ int32_t s_key[kClsHash];
   __shared__ int32_t s_cnt[kClsHash];
@@ -1158,7 +1159,10 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
         }
       }
       const int L = kClassL[cls];
-      const int nss = (nb + L * kRows - 1) / (L * kRows);
+      // start pass: bin by the rows the pair is expected to need (the class,
+      // and the group's super-strips, still follow |B|)
+      const int nb_key = target ? min(nb, 2 * max(target[p], 0) + 16) : nb;
+      const int nss = (nb_key + L * kRows - 1) / (L * kRows);
       bin = (cls * kNssBins + nss_bin_of(nss)) * kMBins + mbin_of(m);
       bin_of[p] = bin;
       atomicAdd(&s_count[cls], 1);
@@ -1669,7 +1673,8 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_cells, dim3(nb), dim3(256), 0, st, n, B.a_len, B.b_len, B.ctl);
   hipLaunchKernelGGL(k_classify, dim3(nblocks(std::max(n, 1), kClsThreads)), dim3(kClsThreads), 0,
                      st, n, B.a_len, B.b_len, B.target_slots, B.plain ? 0 : B.force_class,
-                     B.bin_of, B.rank, B.hist, B.ctl, B.scores, B.a_range, B.b_range);
+                     B.bin_of, B.rank, B.hist, B.ctl, B.scores, B.a_range, B.b_range,
+                     B.plain ? nullptr : B.target);
   if (B.plain) {
     hipLaunchKernelGGL(k_plain_setup, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.sorted);
     hipLaunchKernelGGL(k_plain_groups, dim3(nblocks((int64_t)n * 64 + 64, 256)), dim3(256), 0,
