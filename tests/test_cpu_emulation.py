@@ -122,6 +122,32 @@ def test_shape_knobs_emulated(emu_lib):
     eng.close()
 
 
+def test_last_strip_rows_emulated(emu_lib):
+    """Pipelines of 2, 4 and 16 lanes whose last super-strip needs 1-16 rows per
+    lane (run with 8, 12 or 16), forward and with starts (whose pass stops
+    early), against the oracle."""
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(43)
+    for lanes in (2, 4, 16):
+        A, B = [], []
+        for need in (1, 3, 8, 9, 12, 13, 16):
+            nb = 16 * lanes + lanes * (need - 1) + 1  # one full super-strip, then `need` rows per lane
+            a = "".join(rng.choice(list("ACGT"), int(rng.integers(20, 60))))
+            b = "".join(rng.choice(list("ACGT"), nb - len(a) // 2)) + a[: len(a) // 2]
+            A.append(a)
+            B.append(b[:nb])
+        for fwd in (True, False):
+            eng = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
+                                I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1,
+                                                forwardOnly=fwd))
+            eng.set_lanes_per_pair(lanes)
+            exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1, forward_only=fwd)
+            for g, e in zip(eng.compare(A, B), exp):
+                assert np.array_equal(np.asarray(g), np.asarray(e)), (lanes, fwd)
+            eng.close()
+
+
 def test_request_server_emulated(emu_lib):
     """Three submitting threads, a 2 KiB arena (so admission blocks and the ring
     wraps), results checked against the oracle; then a request larger than the
