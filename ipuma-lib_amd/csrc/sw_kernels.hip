@@ -1404,32 +1404,39 @@ __global__ void k_order_groups(int32_t max_groups, const BucketCtl *ctl, const G
 // forward pass on the reversed prefixes A[0..Aend], B[0..Bend], whose first
 // maximum (i', j') maps back to Bstart = Bend - i', Astart = Aend - j'.
 // ---------------------------------------------------------------------------
-__global__ void k_reverse_prefix(int32_t n, const uint8_t *codes, const int64_t *a_off,
-                                 const int32_t *a_len, const int64_t *b_off,
-                                 const int32_t *b_len, const int32_t *scores,
-                                 const int32_t *a_range, const int32_t *b_range,
-                                 uint8_t *rcodes, int32_t *ra_len, int32_t *rb_len,
-                                 int32_t *error) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  ra_len[p] = 0;
-  rb_len[p] = 0;
-  if (scores[p] <= 0) return;
-  const int la = (int)(((uint32_t)a_range[p] >> 16) & 0xFFFF) + 1;
-  const int lb = (int)(((uint32_t)b_range[p] >> 16) & 0xFFFF) + 1;
-  // an end outside its sequence would write past the reversed copy
-  if (la > a_len[p] || lb > b_len[p]) {
-    atomicOr(error, 2);
-    return;
+// One wave per pair: the lanes copy the reversed prefixes byte by byte, the
+// reads of a wave falling in one span of the sequence and the writes
+// contiguous.
+__global__ __launch_bounds__(256) void k_reverse_prefix(
+    int32_t n, const uint8_t *codes, const int64_t *a_off, const int32_t *a_len,
+    const int64_t *b_off, const int32_t *b_len, const int32_t *scores, const int32_t *a_range,
+    const int32_t *b_range, uint8_t *rcodes, int32_t *ra_len, int32_t *rb_len, int32_t *error) {
+  const int64_t p64 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p64 >= n) return;  // uniform over the wave
+  const int p = (int)p64;
+  int la = 0, lb = 0;
+  if (scores[p] > 0) {
+    la = (int)(((uint32_t)a_range[p] >> 16) & 0xFFFF) + 1;
+    lb = (int)(((uint32_t)b_range[p] >> 16) & 0xFFFF) + 1;
+    // an end outside its sequence would write past the reversed copy
+    if (la > a_len[p] || lb > b_len[p]) {
+      if (lane == 0) atomicOr(error, 2);
+      la = lb = 0;
+    }
   }
-  const uint8_t *a = codes + a_off[p];
-  const uint8_t *b = codes + b_off[p];
-  uint8_t *ra = rcodes + a_off[p];
-  uint8_t *rb = rcodes + b_off[p];
-  for (int t = 0; t < la; ++t) ra[t] = a[la - 1 - t];
-  for (int t = 0; t < lb; ++t) rb[t] = b[lb - 1 - t];
-  ra_len[p] = la;
-  rb_len[p] = lb;
+  if (la > 0) {
+    const uint8_t *a = codes + a_off[p];
+    const uint8_t *b = codes + b_off[p];
+    uint8_t *ra = rcodes + a_off[p];
+    uint8_t *rb = rcodes + b_off[p];
+    for (int t = lane; t < la; t += 64) ra[t] = a[la - 1 - t];
+    for (int t = lane; t < lb; t += 64) rb[t] = b[lb - 1 - t];
+  }
+  if (lane == 0) {
+    ra_len[p] = la;
+    rb_len[p] = lb;
+  }
 }
 
 __global__ void k_finish_starts(int32_t n, const int32_t *scores, const int32_t *ra_range,
@@ -1751,8 +1758,9 @@ hipError_t launch_reverse_prefix(int32_t n, const uint8_t *codes, const int64_t 
                                  int32_t *ra_len, int32_t *rb_len, int32_t *error,
                                  hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_reverse_prefix, dim3(nblocks(n, 128)), dim3(128), 0, st, n, codes, a_off,
-                     a_len, b_off, b_len, scores, a_range, b_range, rcodes, ra_len, rb_len, error);
+  hipLaunchKernelGGL(k_reverse_prefix, dim3(nblocks((int64_t)n * 64, 256)), dim3(256), 0, st, n,
+                     codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, rcodes, ra_len,
+                     rb_len, error);
   return hipGetLastError();
 }
 
