@@ -7,11 +7,14 @@
 // end is the first cell in B-major order (i, then j) holding the maximum.
 //
 // Kernel families
-//   k_fast<KP>     packed int16, two comparisons per lane (low / high halves),
+//   k_fast<KP, STOP>
+//                  packed int16, two comparisons per lane (low / high halves),
 //                  L lanes per comparison pipelined over the B rows (L = 1 is
 //                  the inter-sequence kernel, L = 64 the intra-sequence
 //                  anti-diagonal kernel), persistent over the list of groups
-//                  built by the device bucketing below.
+//                  built by the device bucketing below.  STOP: the start
+//                  pass's instance, which ends a group once every pair has
+//                  reached its forward score.
 //   k_generic32    exact int32 re-statement of the vertex loop, one comparison
 //                  per lane; re-runs int16-saturated comparisons and serves
 //                  gap settings outside the packed kernel's domain.
