@@ -141,6 +141,17 @@ def run_search(args):
         eng.search_device(d["codes"], d["codes"].numel(), d["q_off"], d["q_len"], d["db_off"],
                           d["db_len"], k, scores, ti, ts, stream=sptr)
 
+    tuned = None
+    if args.autotune == "on" and not args.rounds:
+        # bucketing shape chosen on this search, outside the timed region
+        from ipuma_amd import autotune as AT
+        dbs = b["db_len"][:4096]  # pairs of every query with a database sample
+        key = AT.signature(np.repeat(b["q_len"], len(dbs)), np.tile(dbs, nq), "search-blosum62")
+        tuned = AT.tune(eng, step, key=key, reps=2, sync=torch.cuda.synchronize)
+        tab = ", ".join("%dx%.2f: %.1f ms" % (r["rounds"], r["scale"], r["ms"])
+                        for r in tuned["table"])
+        log(f"autotune {key}: rounds {tuned['rounds']} scale {tuned['scale']} ({tab})")
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -204,7 +215,9 @@ def run_search(args):
         "config": {"workload": "one-vs-many search: 8 protein queries (128..4096) x 200,000 database "
                                "sequences lognormal(350, 0.5) clipped 30-5000, BLOSUM62 open 11 "
                                "extend 1, every score + top 100 per query",
-                   "queries": nq, "database": ndb, "cells": cells},
+                   "queries": nq, "database": ndb, "cells": cells,
+                   "autotune": ({"rounds": tuned["rounds"], "step_cap_scale": tuned["scale"],
+                                 "candidates": len(tuned["table"])} if tuned else None)},
         "parity": {"subset_2000_scores": ok_subset, "top100_order": ok_top,
                    "top100_oracle_scores": ok_top_oracle,
                    "planted_homologs_found": f"{found}/{len(planted)}"},
