@@ -65,8 +65,10 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(batch, target_s):
-    """The oracle (C, OpenMP over pairs) on a bounded leading sample of the batch."""
+def cpu_baseline(batch, target_s, gpu_scores=None):
+    """The reference's CPU path (its SSW comparator, compiled from the reference's sources
+    into oracle/_ref) on a bounded leading sample of the batch, OpenMP over pairs; where that
+    library is absent, the oracle (C restatement of the IPU vertex) instead."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
@@ -78,12 +80,20 @@ def cpu_baseline(batch, target_s):
     threads = O.max_threads()
     cells = batch["a_len"].astype(np.int64) * batch["b_len"].astype(np.int64)
     csum = np.cumsum(cells)
+    reference = O.have_ref()
+    out = {}
 
     def run(k):
         t = time.perf_counter()
-        O.sw_batch_codes(batch["codes"], batch["a_off"][:k], batch["a_len"][:k],
-                         batch["b_off"][:k], batch["b_len"][:k], mat, sw["gapInit"],
-                         sw["gapExtend"], True, threads)
+        if reference:
+            out["s"] = O.ssw_batch_codes(batch["codes"], batch["a_off"][:k], batch["a_len"][:k],
+                                         batch["b_off"][:k], batch["b_len"][:k], mat,
+                                         -(sw["gapInit"] + sw["gapExtend"]), -sw["gapExtend"],
+                                         threads)
+        else:
+            out["s"] = O.sw_batch_codes(batch["codes"], batch["a_off"][:k], batch["a_len"][:k],
+                                        batch["b_off"][:k], batch["b_len"][:k], mat,
+                                        sw["gapInit"], sw["gapExtend"], True, threads)[0]
         return time.perf_counter() - t
 
     k0 = int(np.searchsorted(csum, 2e8 * threads / 16)) + 1
@@ -94,10 +104,17 @@ def cpu_baseline(batch, target_s):
     k = min(max(k, k0), len(cells))
     t = run(k)
     g = csum[k - 1] / t / 1e9
-    return {"value": round(float(g), 3), "unit": "GCUPS", "cores": int(threads), "kind": "port",
-            "sample": f"first {k} of {len(cells)} pairs ({int(csum[k - 1])} cells), "
-                      f"forward pass, int32 oracle, {t:.1f} s on {threads} threads "
-                      f"({platform.processor() or platform.machine()})"}
+    what = ("the reference's SSW comparator (cpusw/aligners.hpp: query profile + ssw_align, "
+            "end positions only)" if reference else "int32 oracle")
+    res = {"value": round(float(g), 3), "unit": "GCUPS", "cores": int(threads),
+           "kind": "reference" if reference else "port",
+           "sample": f"first {k} of {len(cells)} pairs ({int(csum[k - 1])} cells), forward "
+                     f"pass, {what}, {t:.1f} s on {threads} threads "
+                     f"({platform.processor() or platform.machine()})"}
+    if gpu_scores is not None:
+        res["scores_equal_gpu"] = bool(np.array_equal(np.asarray(out["s"])[:k],
+                                                      np.asarray(gpu_scores)[:k]))
+    return res
 
 
 def measured_traffic():
@@ -354,7 +371,7 @@ def main():
     cpu = None
     if args.cpu_seconds > 0:
         log("timing the CPU baseline ...")
-        cpu = cpu_baseline(batch, args.cpu_seconds)
+        cpu = cpu_baseline(batch, args.cpu_seconds, gpu_scores=scores.cpu().numpy())
         log(f"cpu baseline: {cpu}")
 
     serving = None

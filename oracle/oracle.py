@@ -93,6 +93,32 @@ def _ptr(a: np.ndarray, t):
     return a.ctypes.data_as(t)
 
 
+def ssw_batch_codes(codes, a_off, a_len, b_off, b_len, mat: np.ndarray, gap_open: int,
+                    gap_extend: int, threads: int) -> np.ndarray:
+    """The reference's CPU comparator (SSW through cpusw/aligners.hpp:128-152: a query
+    profile of A, then ssw_align against B, end positions only) over a coded batch,
+    OpenMP over pairs.  gap_open / gap_extend are SSW's positive penalties (the
+    reference's gapInit = -2, gapExtend = -1 is SSW's 3 / 1).  Scores only."""
+    lib = _ref()
+    lib.ssw_ref_batch_codes.argtypes = [ctypes.c_int32, _u8p, _i64p, _i32p, _i64p, _i32p, _i8p,
+                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_int32, _i32p]
+    lib.ssw_ref_batch_codes.restype = None
+    n = len(a_len)
+    codes = np.ascontiguousarray(codes, np.uint8)
+    a_off = np.ascontiguousarray(a_off, np.int64)
+    b_off = np.ascontiguousarray(b_off, np.int64)
+    a_len = np.ascontiguousarray(a_len, np.int32)
+    b_len = np.ascontiguousarray(b_len, np.int32)
+    mat = np.ascontiguousarray(mat, np.int8)
+    out = np.zeros(n, np.int32)
+    lib.ssw_ref_batch_codes(n, _ptr(codes, _u8p), _ptr(a_off, _i64p), _ptr(a_len, _i32p),
+                            _ptr(b_off, _i64p), _ptr(b_len, _i32p), _ptr(mat, _i8p),
+                            mat.shape[0], int(gap_open), int(gap_extend), int(threads),
+                            _ptr(out, _i32p))
+    return out
+
+
 def select_matrix(similarity: int, match: int = 1, mismatch: int = -1,
                   ambiguity: int = -1) -> np.ndarray:
     """selectMatrix (src/swatlib/similarity.cpp:89-104), as a dim x dim int8."""

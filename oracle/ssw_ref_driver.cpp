@@ -13,6 +13,7 @@
 #include <cstring>
 
 #include "ssw.hpp"
+#include "ssw_core.hpp"
 
 namespace {
 void fill(const StripedSmithWaterman::Alignment &aln, int32_t out[5]) {
@@ -88,5 +89,34 @@ int ssw_ref_align_matrix_cigar(const char *query, int qlen, const char *ref, int
   fill(aln, out);
   cigar_out(aln, cigar, cap);
   return 0;
+}
+
+// The reference's CPU comparator over a whole batch, as its benchmark driver
+// times it (cpusw/aligners.hpp:128-152): per pair a query profile of A
+// (ssw_init, 8- and 16-bit) and ssw_align against B, end positions only
+// (flag 0: the forward-only benchmark), mask max(|A| / 2, 15), pairs spread
+// over `threads` OpenMP threads.  `codes` are residue codes and `mat` is
+// dim x dim in the same code space; gap_open / gap_extend are SSW's positive
+// penalties (first gap residue, each further one).  Scores go to `scores`
+// (-1 where SSW returns no alignment).
+void ssw_ref_batch_codes(int32_t n, const uint8_t *codes, const int64_t *a_off,
+                         const int32_t *a_len, const int64_t *b_off, const int32_t *b_len,
+                         const int8_t *mat, int32_t dim, int32_t gap_open, int32_t gap_extend,
+                         int32_t threads, int32_t *scores) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+  for (int32_t p = 0; p < n; ++p) {
+    const int32_t la = a_len[p], lb = b_len[p];
+    if (la <= 0 || lb <= 0) {
+      scores[p] = 0;
+      continue;
+    }
+    s_profile *prof = ssw_init(reinterpret_cast<const int8_t *>(codes + a_off[p]), la, mat, dim, 2);
+    const int32_t mask = la / 2 > 15 ? la / 2 : 15;
+    s_align *al = ssw_align(prof, reinterpret_cast<const int8_t *>(codes + b_off[p]), lb,
+                            (uint8_t)gap_open, (uint8_t)gap_extend, 0, 0, 0, mask);
+    scores[p] = al ? al->score1 : -1;
+    if (al) align_destroy(al);
+    init_destroy(prof);
+  }
 }
 }

@@ -157,3 +157,22 @@ def test_live_ssw_random_dna():
         assert int(s[0]) == sc
         if sc > 0:
             assert O.unpack_range(ar[0]) == (qb, qe) and O.unpack_range(br[0]) == (rb, re_)
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (needs the reference tree)")
+def test_ssw_batch_scores_equal_oracle():
+    """The reference's SSW comparator over a coded batch (the CPU baseline bench.py
+    times) gives the oracle's scores: protein and DNA, gap settings mapped as SSW's
+    positive open (first gap residue) and extend penalties."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..",
+                                    "ipuma-lib_amd"))
+    from ipuma_amd import workloads as W
+    for b, mat in ((W.config1(n=400, seed=3), O.select_matrix(O.SIM_BLOSUM62)),
+                   (W.config0(seed=4), O.select_matrix(O.SIM_NA, 1, -1, -1))):
+        sw = b["sw"]
+        s = O.ssw_batch_codes(b["codes"], b["a_off"], b["a_len"], b["b_off"], b["b_len"], mat,
+                              -(sw["gapInit"] + sw["gapExtend"]), -sw["gapExtend"], 4)
+        e, _, _ = O.sw_batch_codes(b["codes"], b["a_off"], b["a_len"], b["b_off"], b["b_len"],
+                                   mat, sw["gapInit"], sw["gapExtend"], True)
+        assert np.array_equal(s, e)
