@@ -17,7 +17,9 @@ Three independent checkers live here:
   of src/driver.hpp:47-49 (a gap of length k costs open + (k-1)*extend).
 * ``libssw_ref.so`` (oracle/_ref, built from the reference's vendored SSW by
   oracle/Makefile) -- the reference's own CPU comparator
-  (test/parity-test.cpp:70-87).  Present only where the reference tree is.
+  (test/parity-test.cpp:70-87).  Present only where the reference tree is
+  (built there, the library travels with the tree); bench.py's CPU baseline
+  times it over a batch (``ssw_batch_codes``).
 """
 from __future__ import annotations
 

@@ -5,7 +5,9 @@
 // test/parity-test.cpp:70-87 and cpusw/aligners.hpp:128-152).  oracle/Makefile
 // compiles that library straight from the reference tree together with this
 // file into oracle/_ref/libssw_ref.so; nothing from the reference is copied.
-// tests/ use it to pin the oracle's DP and to freeze golden fixtures.
+// tests/ use it to pin the oracle's DP and to freeze golden fixtures, and
+// bench.py's CPU baseline times it (ssw_ref_batch_codes) as the reference's
+// own CPU path.
 //
 // Orientation follows the reference's parity test: the reference's A is SSW's
 // "query" and its B is SSW's "ref" (test/parity-test.cpp:80, 90-93).
