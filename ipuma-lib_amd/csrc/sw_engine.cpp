@@ -132,7 +132,8 @@ struct ipuma_engine {
   DevBuf s_pairs, s_res;  // one-vs-many search: pair descriptors and results
   // host staging of compare_local / compare_mn_local
   DevBuf in_chars, in_codes, in_meta, out_res;
-  std::vector<char> h_chars;
+  char *h_pin = nullptr;  // pinned staging of host characters (grown as needed)
+  size_t h_pin_cap = 0;
   std::vector<int64_t> h_aoff, h_boff;
   std::vector<int32_t> h_alen, h_blen;
   ipuma_batch_stats stats{};
@@ -373,13 +374,42 @@ int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
     pos += (lens[i] + 3) / 4 * 4;
   }
   total = pos + 16;
-  e->h_chars.assign((size_t)total, 0);
-  for (size_t i = 0; i < ptrs.size(); ++i)
-    if (lens[i]) std::memcpy(e->h_chars.data() + offs[i], ptrs[i], (size_t)lens[i]);
   int rc;
   if ((rc = e->in_chars.ensure((size_t)total)) || (rc = e->in_codes.ensure((size_t)total))) return rc;
-  HIP_TRY(hipMemcpyAsync(e->in_chars.p, e->h_chars.data(), (size_t)total, hipMemcpyHostToDevice,
-                         e->stream));
+  // The characters go through a pinned staging buffer (one DMA at full link
+  // speed instead of the runtime's pageable path); the previous call's copy
+  // out of it has finished before it is overwritten.
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if ((size_t)total > e->h_pin_cap) {
+    if (e->h_pin) (void)hipHostFree(e->h_pin);
+    e->h_pin = nullptr;
+    e->h_pin_cap = 0;
+    const size_t cap = ((size_t)total + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+    HIP_TRY(hipHostMalloc(&e->h_pin, cap, hipHostMallocDefault));
+    e->h_pin_cap = cap;
+  }
+  char *const dst = e->h_pin;
+  // sequences at 4-byte aligned offsets, the bytes between them and the tail zero
+  auto copy_range = [&](size_t i0, size_t i1) {
+    for (size_t i = i0; i < i1; ++i) {
+      if (lens[i]) std::memcpy(dst + offs[i], ptrs[i], (size_t)lens[i]);
+      const int64_t end = (i + 1 < ptrs.size()) ? offs[i + 1] : pos;
+      if (end > offs[i] + lens[i]) std::memset(dst + offs[i] + lens[i], 0, (size_t)(end - offs[i] - lens[i]));
+    }
+  };
+  const size_t np = ptrs.size();
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nth = total > ((int64_t)8 << 20) ? std::min<size_t>(std::min(8u, hw), np) : 1;
+  if (nth > 1) {
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nth; ++t)
+      th.emplace_back(copy_range, np * t / nth, np * (t + 1) / nth);
+    for (auto &x : th) x.join();
+  } else {
+    copy_range(0, np);
+  }
+  std::memset(dst + pos, 0, 16);
+  HIP_TRY(hipMemcpyAsync(e->in_chars.p, dst, (size_t)total, hipMemcpyHostToDevice, e->stream));
   HIP_TRY(launch_encode(e->in_chars.as<uint8_t>(), total, e->in_codes.as<uint8_t>(), e->d_codetab,
                         e->stream));
   return IPUMA_OK;
@@ -635,6 +665,9 @@ void ipuma_engine_destroy(ipuma_engine *e) {
     e->pj_thread.join();
   }
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->h_pin) (void)hipHostFree(e->h_pin);
+  e->h_pin = nullptr;
+  e->h_pin_cap = 0;
   e->scratch.release();
   e->band_scratch.release();
   e->work.release();
