@@ -398,18 +398,31 @@ int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
     }
   };
   const size_t np = ptrs.size();
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nth = total > ((int64_t)8 << 20) ? std::min<size_t>(std::min(8u, hw), np) : 1;
-  if (nth > 1) {
-    std::vector<std::thread> th;
-    for (size_t t = 0; t < nth; ++t)
-      th.emplace_back(copy_range, np * t / nth, np * (t + 1) / nth);
-    for (auto &x : th) x.join();
-  } else {
-    copy_range(0, np);
-  }
   std::memset(dst + pos, 0, 16);
-  HIP_TRY(hipMemcpyAsync(e->in_chars.p, dst, (size_t)total, hipMemcpyHostToDevice, e->stream));
+  if (total <= ((int64_t)8 << 20)) {
+    copy_range(0, np);
+    HIP_TRY(hipMemcpyAsync(e->in_chars.p, dst, (size_t)total, hipMemcpyHostToDevice, e->stream));
+  } else {
+    // large batches: four slices, each gathered by up to eight threads and sent
+    // while the next one is gathered
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nth = std::min<size_t>(std::min(8u, hw), np);
+    constexpr size_t kSlices = 4;
+    for (size_t sl = 0; sl < kSlices; ++sl) {
+      const size_t i0 = np * sl / kSlices, i1 = np * (sl + 1) / kSlices;
+      if (i0 == i1) continue;
+      std::vector<std::thread> th;
+      for (size_t t = 0; t < nth; ++t) {
+        const size_t a = i0 + (i1 - i0) * t / nth, b = i0 + (i1 - i0) * (t + 1) / nth;
+        if (a < b) th.emplace_back(copy_range, a, b);
+      }
+      for (auto &x : th) x.join();
+      const int64_t b0 = offs[i0];
+      const int64_t b1 = sl + 1 == kSlices ? total : offs[i1];
+      HIP_TRY(hipMemcpyAsync(e->in_chars.as<char>() + b0, dst + b0, (size_t)(b1 - b0),
+                             hipMemcpyHostToDevice, e->stream));
+    }
+  }
   HIP_TRY(launch_encode(e->in_chars.as<uint8_t>(), total, e->in_codes.as<uint8_t>(), e->d_codetab,
                         e->stream));
   return IPUMA_OK;

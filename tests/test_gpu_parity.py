@@ -194,6 +194,31 @@ def test_protein_slice_forward(gpu):
     assert_same(device_run(eng, batch), oracle_batch(batch, True), "protein slice")
 
 
+def test_host_staging_large(gpu):
+    # host-buffer entry on a batch above the 8 MB line where the pinned staging
+    # copy is gathered in slices by several threads, each slice sent while the
+    # next is gathered: same results as the device-resident entry, and the first
+    # 1500 pairs against the oracle
+    note("host-buffer staging: protein batch of 30000 pairs (about 20 MB)")
+    batch = W.config1(n=30000)
+    letters = np.frombuffer(b"ARNDCQEGHILKMFPSTWYVBZX*", np.uint8)
+    dec = np.zeros(256, np.uint8)
+    dec[: len(letters)] = letters
+    codes = batch["codes"]
+    A = [dec[codes[o:o + l]].tobytes() for o, l in zip(batch["a_off"], batch["a_len"])]
+    B = [dec[codes[o:o + l]].tobytes() for o, l in zip(batch["b_off"], batch["b_len"])]
+    assert sum(map(len, A)) + sum(map(len, B)) > 8 << 20
+    eng = engine_for(batch["sw"], False)
+    got = eng.compare(A, B)
+    assert_same(got, device_run(eng, batch), "host staging vs device entry")
+    sw = batch["sw"]
+    exp = O.sw_batch([a.decode() for a in A[:1500]], [b.decode() for b in B[:1500]],
+                     similarity=SIM[sw["similarity"]], datatype=DT[sw["datatype"]],
+                     gap_init=sw["gapInit"], gap_extend=sw["gapExtend"])
+    assert_same([np.asarray(x)[:1500] for x in got], exp, "host staging vs oracle")
+    eng.close()
+
+
 def test_long_pairs(gpu):
     note("long DNA pairs (2k-6k) with starts")
     rng = np.random.default_rng(9)
