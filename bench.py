@@ -65,6 +65,18 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo (else the platform's machine name)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(batch, target_s, gpu_scores=None):
     """The reference's CPU path (its SSW comparator, compiled from the reference's sources
     into oracle/_ref) on a bounded leading sample of the batch, OpenMP over pairs; where that
@@ -110,7 +122,7 @@ def cpu_baseline(batch, target_s, gpu_scores=None):
            "kind": "reference" if reference else "port",
            "sample": f"first {k} of {len(cells)} pairs ({int(csum[k - 1])} cells), forward "
                      f"pass, {what}, {t:.1f} s on {threads} threads "
-                     f"({platform.processor() or platform.machine()})"}
+                     f"of {os.cpu_count()} visible ({cpu_model()})"}
     if gpu_scores is not None:
         res["scores_equal_gpu"] = bool(np.array_equal(np.asarray(out["s"])[:k],
                                                       np.asarray(gpu_scores)[:k]))

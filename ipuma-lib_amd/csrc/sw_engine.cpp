@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <condition_variable>
 #include <cstdarg>
@@ -403,25 +404,37 @@ int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
     copy_range(0, np);
     HIP_TRY(hipMemcpyAsync(e->in_chars.p, dst, (size_t)total, hipMemcpyHostToDevice, e->stream));
   } else {
-    // large batches: four slices, each gathered by up to eight threads and sent
-    // while the next one is gathered
+    // large batches: eight slices, each gathered by up to eight threads (this one
+    // among them) and sent as soon as all have finished it, while the threads go
+    // on to the next slice
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t nth = std::min<size_t>(std::min(8u, hw), np);
-    constexpr size_t kSlices = 4;
-    for (size_t sl = 0; sl < kSlices; ++sl) {
+    const size_t nth = std::max<size_t>(1, std::min<size_t>(std::min(8u, hw), np));
+    constexpr size_t kSlices = 8;
+    std::atomic<size_t> done[kSlices];
+    for (auto &d : done) d.store(0, std::memory_order_relaxed);
+    auto part = [&](size_t t, size_t sl) {
       const size_t i0 = np * sl / kSlices, i1 = np * (sl + 1) / kSlices;
-      if (i0 == i1) continue;
-      std::vector<std::thread> th;
-      for (size_t t = 0; t < nth; ++t) {
-        const size_t a = i0 + (i1 - i0) * t / nth, b = i0 + (i1 - i0) * (t + 1) / nth;
-        if (a < b) th.emplace_back(copy_range, a, b);
-      }
-      for (auto &x : th) x.join();
+      copy_range(i0 + (i1 - i0) * t / nth, i0 + (i1 - i0) * (t + 1) / nth);
+      done[sl].fetch_add(1, std::memory_order_release);
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nth; ++t)
+      th.emplace_back([&part, t] {
+        for (size_t sl = 0; sl < kSlices; ++sl) part(t, sl);
+      });
+    hipError_t err = hipSuccess;
+    for (size_t sl = 0; sl < kSlices; ++sl) {
+      part(0, sl);
+      while (done[sl].load(std::memory_order_acquire) < nth) std::this_thread::yield();
+      const size_t i0 = np * sl / kSlices, i1 = np * (sl + 1) / kSlices;
+      if (i0 == i1 || err != hipSuccess) continue;
       const int64_t b0 = offs[i0];
       const int64_t b1 = sl + 1 == kSlices ? total : offs[i1];
-      HIP_TRY(hipMemcpyAsync(e->in_chars.as<char>() + b0, dst + b0, (size_t)(b1 - b0),
-                             hipMemcpyHostToDevice, e->stream));
+      err = hipMemcpyAsync(e->in_chars.as<char>() + b0, dst + b0, (size_t)(b1 - b0),
+                           hipMemcpyHostToDevice, e->stream);
     }
+    for (auto &x : th) x.join();
+    HIP_TRY(err);
   }
   HIP_TRY(launch_encode(e->in_chars.as<uint8_t>(), total, e->in_codes.as<uint8_t>(), e->d_codetab,
                         e->stream));
