@@ -659,7 +659,7 @@ void ipuma_default_algo_config(ipuma_algo_config *c) {
 
 const char *ipuma_last_error(void) { return g_last_error.c_str(); }
 
-int ipuma_engine_create(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
+static int ipuma_engine_create_body(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
                         ipuma_engine **out) {
   if (!out) return fail(IPUMA_EINVAL, "out is NULL");
   *out = nullptr;
@@ -726,7 +726,7 @@ void ipuma_engine_destroy(ipuma_engine *e) {
   delete e;
 }
 
-int ipuma_compare_local(ipuma_engine *e, int32_t n, const char *a_chars, const int64_t *a_offsets,
+static int ipuma_compare_local_body(ipuma_engine *e, int32_t n, const char *a_chars, const int64_t *a_offsets,
                         const char *b_chars, const int64_t *b_offsets, int32_t *scores,
                         int32_t *a_range, int32_t *b_range) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
@@ -762,7 +762,7 @@ int ipuma_compare_local(ipuma_engine *e, int32_t n, const char *a_chars, const i
   return run_host_batch(e, n, total, scores, a_range, b_range);
 }
 
-int ipuma_compare_mn_local(ipuma_engine *e, int32_t nseqs, const char *chars,
+static int ipuma_compare_mn_local_body(ipuma_engine *e, int32_t nseqs, const char *chars,
                            const int64_t *offsets, int32_t ncmps, const int32_t *cmps,
                            int32_t *scores, int32_t *a_range, int32_t *b_range) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
@@ -801,7 +801,7 @@ int ipuma_compare_mn_local(ipuma_engine *e, int32_t nseqs, const char *chars,
   return run_host_batch(e, ncmps, total, scores, a_range, b_range);
 }
 
-int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
+static int ipuma_compare_device_body(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
                          const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
                          const int32_t *b_len, int32_t *scores, int32_t *a_range,
                          int32_t *b_range, void *stream) {
@@ -815,7 +815,7 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
                              b_range, st);
 }
 
-int ipuma_search_device(ipuma_engine *e, const uint8_t *codes, int64_t codes_len, int32_t nq,
+static int ipuma_search_device_body(ipuma_engine *e, const uint8_t *codes, int64_t codes_len, int32_t nq,
                         const int64_t *q_off, const int32_t *q_len, int32_t ndb,
                         const int64_t *db_off, const int32_t *db_len, int32_t k,
                         int32_t *scores, int32_t *top_idx, int32_t *top_score, void *stream) {
@@ -917,7 +917,7 @@ int ipuma_set_lanes_per_pair(ipuma_engine *e, int32_t lanes) {
   return fail(IPUMA_EINVAL, "lanes per pair must be 0, 1, 2, 4, 8, 16 or 64");
 }
 
-int ipuma_encode_device(ipuma_engine *e, const char *d_chars, int64_t len, uint8_t *d_codes,
+static int ipuma_encode_device_body(ipuma_engine *e, const char *d_chars, int64_t len, uint8_t *d_codes,
                         void *stream) {
   if (!e || len < 0 || (len > 0 && (!d_chars || !d_codes))) return fail(IPUMA_EINVAL, "bad args");
   std::lock_guard<std::mutex> lock_(e->mu);
@@ -926,7 +926,7 @@ int ipuma_encode_device(ipuma_engine *e, const char *d_chars, int64_t len, uint8
   return IPUMA_OK;
 }
 
-int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out) {
+static int ipuma_last_batch_stats_body(ipuma_engine *e, ipuma_batch_stats *out) {
   if (!e || !out) return fail(IPUMA_EINVAL, "bad args");
   std::lock_guard<std::mutex> lock_(e->mu);
   std::memset(out, 0, sizeof(*out));
@@ -956,14 +956,14 @@ int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out) {
   return IPUMA_OK;
 }
 
-int ipuma_prepared_compare(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
+static int ipuma_prepared_compare_body(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
                            int32_t *results, int64_t result_words, int32_t lot) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
   std::lock_guard<std::mutex> lock_(e->mu);
   return prepared_impl(e, inputs, input_words, results, result_words, lot);
 }
 
-int ipuma_prepared_submit(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
+static int ipuma_prepared_submit_body(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
                           int32_t *results, int64_t result_words, int64_t *job) {
   if (!e || !job) return fail(IPUMA_EINVAL, "engine or job is NULL");
   if (input_words < 0 || result_words < 0 || (input_words > 0 && !inputs) ||
@@ -984,7 +984,7 @@ int ipuma_prepared_submit(ipuma_engine *e, const int32_t *inputs, int64_t input_
   return IPUMA_OK;
 }
 
-int ipuma_prepared_join(ipuma_engine *e, int64_t job) {
+static int ipuma_prepared_join_body(ipuma_engine *e, int64_t job) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
   std::unique_lock<std::mutex> lk(e->pj_mu);
   auto it = e->pj_jobs.find(job);
@@ -1001,7 +1001,7 @@ int ipuma_prepared_join(ipuma_engine *e, int64_t job) {
   return IPUMA_OK;
 }
 
-int ipuma_traceback_local(ipuma_engine *e, int32_t n, const char *a_chars,
+static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_chars,
                           const int64_t *a_offsets, const char *b_chars, const int64_t *b_offsets,
                           const int32_t *scores, const int32_t *a_range, const int32_t *b_range,
                           uint32_t *cigar, const int64_t *cigar_offsets, int32_t *cigar_len) {
@@ -1129,6 +1129,68 @@ int ipuma_synchronize(ipuma_engine *e) {
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipDeviceSynchronize());
   return check_device_error(e);
+}
+
+// Exported entry points: the bodies above, with no exception crossing the ABI.
+int ipuma_engine_create(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
+                        ipuma_engine **out) {
+  return ipuma::no_throw([&] { return ipuma_engine_create_body(sw, algo, out); });
+}
+
+int ipuma_compare_local(ipuma_engine *e, int32_t n, const char *a_chars, const int64_t *a_offsets,
+                        const char *b_chars, const int64_t *b_offsets, int32_t *scores,
+                        int32_t *a_range, int32_t *b_range) {
+  return ipuma::no_throw([&] { return ipuma_compare_local_body(e, n, a_chars, a_offsets, b_chars, b_offsets, scores, a_range, b_range); });
+}
+
+int ipuma_compare_mn_local(ipuma_engine *e, int32_t nseqs, const char *chars,
+                           const int64_t *offsets, int32_t ncmps, const int32_t *cmps,
+                           int32_t *scores, int32_t *a_range, int32_t *b_range) {
+  return ipuma::no_throw([&] { return ipuma_compare_mn_local_body(e, nseqs, chars, offsets, ncmps, cmps, scores, a_range, b_range); });
+}
+
+int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
+                         const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
+                         const int32_t *b_len, int32_t *scores, int32_t *a_range,
+                         int32_t *b_range, void *stream) {
+  return ipuma::no_throw([&] { return ipuma_compare_device_body(e, n, codes, codes_len, a_off, a_len, b_off, b_len, scores, a_range, b_range, stream); });
+}
+
+int ipuma_search_device(ipuma_engine *e, const uint8_t *codes, int64_t codes_len, int32_t nq,
+                        const int64_t *q_off, const int32_t *q_len, int32_t ndb,
+                        const int64_t *db_off, const int32_t *db_len, int32_t k,
+                        int32_t *scores, int32_t *top_idx, int32_t *top_score, void *stream) {
+  return ipuma::no_throw([&] { return ipuma_search_device_body(e, codes, codes_len, nq, q_off, q_len, ndb, db_off, db_len, k, scores, top_idx, top_score, stream); });
+}
+
+int ipuma_prepared_compare(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
+                           int32_t *results, int64_t result_words, int32_t lot) {
+  return ipuma::no_throw([&] { return ipuma_prepared_compare_body(e, inputs, input_words, results, result_words, lot); });
+}
+
+int ipuma_prepared_submit(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
+                          int32_t *results, int64_t result_words, int64_t *job) {
+  return ipuma::no_throw([&] { return ipuma_prepared_submit_body(e, inputs, input_words, results, result_words, job); });
+}
+
+int ipuma_prepared_join(ipuma_engine *e, int64_t job) {
+  return ipuma::no_throw([&] { return ipuma_prepared_join_body(e, job); });
+}
+
+int ipuma_traceback_local(ipuma_engine *e, int32_t n, const char *a_chars,
+                          const int64_t *a_offsets, const char *b_chars, const int64_t *b_offsets,
+                          const int32_t *scores, const int32_t *a_range, const int32_t *b_range,
+                          uint32_t *cigar, const int64_t *cigar_offsets, int32_t *cigar_len) {
+  return ipuma::no_throw([&] { return ipuma_traceback_local_body(e, n, a_chars, a_offsets, b_chars, b_offsets, scores, a_range, b_range, cigar, cigar_offsets, cigar_len); });
+}
+
+int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out) {
+  return ipuma::no_throw([&] { return ipuma_last_batch_stats_body(e, out); });
+}
+
+int ipuma_encode_device(ipuma_engine *e, const char *d_chars, int64_t len, uint8_t *d_codes,
+                        void *stream) {
+  return ipuma::no_throw([&] { return ipuma_encode_device_body(e, d_chars, len, d_codes, stream); });
 }
 
 }  // extern "C"

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <exception>
+#include <new>
+
 #include "../../include/ipuma_hip.h"
 
 namespace ipuma {
@@ -15,4 +18,19 @@ hipStream_t engine_stream(const ipuma_engine *e);
 int engine_check_error(ipuma_engine *e);
 // Sets the calling thread's last-error message; returns `code`.
 int set_last_error(int code, const char *fmt, ...);
+// Runs an entry point's body and turns a C++ exception escaping it (a failed
+// host allocation, a thread that could not start) into an error code, so no
+// exception crosses the C ABI.
+template <class F>
+int no_throw(F &&body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return set_last_error(IPUMA_ENOMEM, "host allocation failed");
+  } catch (const std::exception &x) {
+    return set_last_error(IPUMA_EDEVICE, "internal: %s", x.what());
+  } catch (...) {
+    return set_last_error(IPUMA_EDEVICE, "internal: unknown exception");
+  }
+}
 }  // namespace ipuma

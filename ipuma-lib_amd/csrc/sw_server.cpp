@@ -309,7 +309,7 @@ void ipuma_default_server_config(ipuma_server_config *c) {
   c->merge_requests = 1;
 }
 
-int ipuma_server_create(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
+static int ipuma_server_create_body(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
                         const ipuma_server_config *cfg, ipuma_server **out) {
   if (!out) return set_last_error(IPUMA_EINVAL, "out is NULL");
   *out = nullptr;
@@ -373,7 +373,7 @@ void ipuma_server_destroy(ipuma_server *s) {
   delete s;
 }
 
-int ipuma_server_submit(ipuma_server *s, int32_t n, const char *a_chars, const int64_t *a_offsets,
+static int ipuma_server_submit_body(ipuma_server *s, int32_t n, const char *a_chars, const int64_t *a_offsets,
                         const char *b_chars, const int64_t *b_offsets, int64_t *ticket) {
   if (!s || !ticket) return set_last_error(IPUMA_EINVAL, "server or ticket is NULL");
   *ticket = 0;
@@ -491,7 +491,7 @@ int ipuma_server_submit(ipuma_server *s, int32_t n, const char *a_chars, const i
   return IPUMA_OK;
 }
 
-int ipuma_server_wait(ipuma_server *s, int64_t ticket, int32_t *scores, int32_t *a_range,
+static int ipuma_server_wait_body(ipuma_server *s, int64_t ticket, int32_t *scores, int32_t *a_range,
                       int32_t *b_range, double *latency_ms) {
   if (!s) return set_last_error(IPUMA_EINVAL, "server is NULL");
   std::unique_lock<std::mutex> lk(s->mu);
@@ -512,7 +512,7 @@ int ipuma_server_wait(ipuma_server *s, int64_t ticket, int32_t *scores, int32_t 
   return st == 2 ? IPUMA_OK : set_last_error(st, "request %lld failed", (long long)ticket);
 }
 
-int ipuma_server_get_stats(ipuma_server *s, ipuma_server_stats *out) {
+static int ipuma_server_get_stats_body(ipuma_server *s, ipuma_server_stats *out) {
   if (!s || !out) return set_last_error(IPUMA_EINVAL, "bad args");
   std::lock_guard<std::mutex> lk(s->mu);
   *out = s->st;
@@ -528,6 +528,26 @@ int ipuma_server_engine(ipuma_server *s, ipuma_engine **out) {
   if (!s || !out) return set_last_error(IPUMA_EINVAL, "bad args");
   *out = s->e;
   return IPUMA_OK;
+}
+
+// Exported entry points: the bodies above, with no exception crossing the ABI.
+int ipuma_server_create(const ipuma_sw_config *sw, const ipuma_algo_config *algo,
+                        const ipuma_server_config *cfg, ipuma_server **out) {
+  return ipuma::no_throw([&] { return ipuma_server_create_body(sw, algo, cfg, out); });
+}
+
+int ipuma_server_submit(ipuma_server *s, int32_t n, const char *a_chars, const int64_t *a_offsets,
+                        const char *b_chars, const int64_t *b_offsets, int64_t *ticket) {
+  return ipuma::no_throw([&] { return ipuma_server_submit_body(s, n, a_chars, a_offsets, b_chars, b_offsets, ticket); });
+}
+
+int ipuma_server_wait(ipuma_server *s, int64_t ticket, int32_t *scores, int32_t *a_range,
+                      int32_t *b_range, double *latency_ms) {
+  return ipuma::no_throw([&] { return ipuma_server_wait_body(s, ticket, scores, a_range, b_range, latency_ms); });
+}
+
+int ipuma_server_get_stats(ipuma_server *s, ipuma_server_stats *out) {
+  return ipuma::no_throw([&] { return ipuma_server_get_stats_body(s, out); });
 }
 
 }  // extern "C"
