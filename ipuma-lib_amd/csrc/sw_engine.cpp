@@ -135,6 +135,8 @@ struct ipuma_engine {
   DevBuf in_chars, in_codes, in_meta, out_res;
   char *h_pin = nullptr;  // pinned staging of host characters (grown as needed)
   size_t h_pin_cap = 0;
+  char *h_meta = nullptr;  // pinned staging of descriptors in, results out
+  size_t h_meta_cap = 0;
   std::vector<int64_t> h_aoff, h_boff;
   std::vector<int32_t> h_alen, h_blen;
   ipuma_batch_stats stats{};
@@ -366,6 +368,19 @@ int compare_device_chain(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
 }
 
 // Host strings -> 4-aligned device arena of codes (encoded on the device).
+// Grows a pinned host buffer to hold `need` bytes (in 1 MB steps); its old
+// contents are not kept.  The caller has synchronized the stream that reads it.
+int ensure_pinned(char *&p, size_t &cap, size_t need) {
+  if (need <= cap) return IPUMA_OK;
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t c = (need + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+  HIP_TRY(hipHostMalloc(&p, c, hipHostMallocDefault));
+  cap = c;
+  return IPUMA_OK;
+}
+
 int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
                 const std::vector<int64_t> &lens, std::vector<int64_t> &offs, int64_t &total) {
   offs.resize(ptrs.size());
@@ -381,14 +396,7 @@ int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
   // speed instead of the runtime's pageable path); the previous call's copy
   // out of it has finished before it is overwritten.
   HIP_TRY(hipStreamSynchronize(e->stream));
-  if ((size_t)total > e->h_pin_cap) {
-    if (e->h_pin) (void)hipHostFree(e->h_pin);
-    e->h_pin = nullptr;
-    e->h_pin_cap = 0;
-    const size_t cap = ((size_t)total + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
-    HIP_TRY(hipHostMalloc(&e->h_pin, cap, hipHostMallocDefault));
-    e->h_pin_cap = cap;
-  }
+  if ((rc = ensure_pinned(e->h_pin, e->h_pin_cap, (size_t)total))) return rc;
   char *const dst = e->h_pin;
   // sequences at 4-byte aligned offsets, the bytes between them and the tail zero
   auto copy_range = [&](size_t i0, size_t i1) {
@@ -468,22 +476,34 @@ int run_host_batch(ipuma_engine *e, int32_t n, int64_t codes_len, int32_t *score
   int64_t *d_boff = d_aoff + n;
   int32_t *d_alen = reinterpret_cast<int32_t *>(d_boff + n);
   int32_t *d_blen = d_alen + n;
+  // descriptors in and results out go through one pinned buffer (one DMA each
+  // way); the stream was synchronized when the previous call returned
+  const size_t res_bytes = (size_t)n * 12;
+  if ((rc = ensure_pinned(e->h_meta, e->h_meta_cap, std::max(meta_bytes, res_bytes) + 16)))
+    return rc;
   if (n > 0) {
-    HIP_TRY(hipMemcpyAsync(d_aoff, e->h_aoff.data(), (size_t)n * 8, hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(d_boff, e->h_boff.data(), (size_t)n * 8, hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(d_alen, e->h_alen.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(d_blen, e->h_blen.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream));
+    char *h = e->h_meta;
+    std::memcpy(h, e->h_aoff.data(), (size_t)n * 8);
+    std::memcpy(h + (size_t)n * 8, e->h_boff.data(), (size_t)n * 8);
+    std::memcpy(h + (size_t)n * 16, e->h_alen.data(), (size_t)n * 4);
+    std::memcpy(h + (size_t)n * 20, e->h_blen.data(), (size_t)n * 4);
+    HIP_TRY(hipMemcpyAsync(m, h, meta_bytes, hipMemcpyHostToDevice, e->stream));
   }
   int32_t *d_s = e->out_res.as<int32_t>(), *d_ar = d_s + n, *d_br = d_s + 2 * (size_t)n;
   rc = compare_device_impl(e, n, e->in_codes.as<uint8_t>(), codes_len, d_aoff, d_alen, d_boff,
                            d_blen, d_s, d_ar, d_br, e->stream);
-  if (rc) return rc;
-  if (n > 0) {
-    HIP_TRY(hipMemcpyAsync(scores, d_s, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipMemcpyAsync(a_range, d_ar, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipMemcpyAsync(b_range, d_br, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
+  if (rc) {
+    (void)hipStreamSynchronize(e->stream);  // the descriptor copy may still read h_meta
+    return rc;
   }
+  if (n > 0) HIP_TRY(hipMemcpyAsync(e->h_meta, d_s, res_bytes, hipMemcpyDeviceToHost, e->stream));
   HIP_TRY(hipStreamSynchronize(e->stream));
+  if (n > 0) {
+    const int32_t *r = reinterpret_cast<const int32_t *>(e->h_meta);
+    std::memcpy(scores, r, (size_t)n * 4);
+    std::memcpy(a_range, r + n, (size_t)n * 4);
+    std::memcpy(b_range, r + 2 * (size_t)n, (size_t)n * 4);
+  }
   return check_device_error(e);
 }
 
@@ -694,6 +714,9 @@ void ipuma_engine_destroy(ipuma_engine *e) {
   if (e->h_pin) (void)hipHostFree(e->h_pin);
   e->h_pin = nullptr;
   e->h_pin_cap = 0;
+  if (e->h_meta) (void)hipHostFree(e->h_meta);
+  e->h_meta = nullptr;
+  e->h_meta_cap = 0;
   e->scratch.release();
   e->band_scratch.release();
   e->work.release();
