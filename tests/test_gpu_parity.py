@@ -187,6 +187,44 @@ def test_skewed_slice_all_modes(gpu):
         assert_same(device_run(eng, batch), exp, f"skewed mode={mode}")
 
 
+def subset(batch, idx):
+    return dict(batch, a_off=batch["a_off"][idx], a_len=batch["a_len"][idx],
+                b_off=batch["b_off"][idx], b_len=batch["b_len"][idx])
+
+
+def test_protein_full_size_with_starts(gpu):
+    # the whole 100,000-pair protein batch (1.29e10 cells), scores and both
+    # ranges of every pair against the oracle
+    note("protein batch at full size (100,000 pairs) with starts vs oracle")
+    batch = W.config1()
+    assert len(batch["a_len"]) == 100_000
+    eng = engine_for(batch["sw"], False)
+    assert_same(device_run(eng, batch), oracle_batch(batch, False), "protein full size")
+    eng.close()
+
+
+def test_skewed_full_size(gpu):
+    # the whole skewed DNA batch (1,000,000 short + 5,000 long pairs): the
+    # bucketed and the plain launch agree on every pair, and the 40 longest
+    # pairs plus 3,000 random short ones agree with the oracle
+    note("skewed DNA batch at full size (1,005,000 pairs): bucketed vs plain vs oracle sample")
+    batch = W.config2()
+    n = len(batch["a_len"])
+    assert n == 1_005_000
+    eng = engine_for(batch["sw"], True)
+    got = device_run(eng, batch)
+    eng.set_batching(I.BATCH_PLAIN)
+    assert_same(device_run(eng, batch), got, "skewed plain vs bucketed")
+    eng.close()
+    cells = batch["a_len"].astype(np.int64) * batch["b_len"].astype(np.int64)
+    longest = np.argsort(cells)[-40:]
+    short = np.random.default_rng(41).choice(np.nonzero(batch["a_len"] == 150)[0], 3000,
+                                             replace=False)
+    idx = np.concatenate([longest, short])
+    exp = oracle_batch(subset(batch, idx), True)
+    assert_same([np.asarray(x)[idx] for x in got], exp, "skewed sample vs oracle")
+
+
 def test_protein_slice_forward(gpu):
     note("protein config slice (4000 pairs) forward only")
     batch = W.config1(n=4000)
