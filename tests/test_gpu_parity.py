@@ -557,3 +557,42 @@ def test_search_top100(gpu):
         assert np.array_equal(ti.cpu().numpy().reshape(nq, k)[q], order), q
         assert np.array_equal(ts.cpu().numpy().reshape(nq, k)[q], exp[order]), q
     eng.close()
+
+
+def test_search_full_size(gpu):
+    # the whole one-vs-many search (8 queries x 200,000 database sequences):
+    # every query's top 100 is the (score desc, index asc) order of its own
+    # 200,000 scores; 1,000 random database sequences per query and every
+    # top-100 entry agree with the oracle
+    note("one-vs-many search at full size: 8 queries x 200,000 sequences, top 100")
+    import torch
+    b = W.config4()
+    nq, ndb, k = len(b["q_len"]), len(b["db_len"]), 100
+    assert (nq, ndb) == (8, 200_000)
+    dev = torch.device("cuda:0")
+    t = {x: torch.from_numpy(np.ascontiguousarray(b[x])).to(dev)
+         for x in ("codes", "q_off", "q_len", "db_off", "db_len")}
+    scores = torch.empty(nq * ndb, dtype=torch.int32, device=dev)
+    ti = torch.empty(nq * k, dtype=torch.int32, device=dev)
+    ts = torch.empty(nq * k, dtype=torch.int32, device=dev)
+    eng = I.SWAlgorithm(I.SWConfig(**W.PROTEIN_SW),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    eng.search_device(t["codes"], t["codes"].numel(), t["q_off"], t["q_len"], t["db_off"],
+                      t["db_len"], k, scores, ti, ts)
+    eng.synchronize()
+    got = scores.cpu().numpy().reshape(nq, ndb)
+    top_i = ti.cpu().numpy().reshape(nq, k)
+    top_s = ts.cpu().numpy().reshape(nq, k)
+    eng.close()
+    mat = O.select_matrix(O.SIM_BLOSUM62)
+    rng = np.random.default_rng(44)
+    for q in range(nq):
+        order = np.lexsort((np.arange(ndb), -got[q].astype(np.int64)))[:k]
+        assert np.array_equal(top_i[q], order), q
+        assert np.array_equal(top_s[q], got[q][order]), q
+        idx = np.concatenate([rng.choice(ndb, 1000, replace=False), order])
+        exp, _, _ = O.sw_batch_codes(b["codes"], b["db_off"][idx], b["db_len"][idx],
+                                     np.full(idx.size, b["q_off"][q]),
+                                     np.full(idx.size, b["q_len"][q], np.int32),
+                                     mat, -10, -1, True)
+        assert np.array_equal(got[q][idx], exp), q
