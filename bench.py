@@ -95,17 +95,17 @@ def cpu_baseline(batch, target_s, gpu_scores=None):
     reference = O.have_ref()
     out = {}
 
-    def run(k):
+    def run(k, nthreads=threads):
         t = time.perf_counter()
         if reference:
             out["s"] = O.ssw_batch_codes(batch["codes"], batch["a_off"][:k], batch["a_len"][:k],
                                          batch["b_off"][:k], batch["b_len"][:k], mat,
                                          -(sw["gapInit"] + sw["gapExtend"]), -sw["gapExtend"],
-                                         threads)
+                                         nthreads)
         else:
             out["s"] = O.sw_batch_codes(batch["codes"], batch["a_off"][:k], batch["a_len"][:k],
                                         batch["b_off"][:k], batch["b_len"][:k], mat,
-                                        sw["gapInit"], sw["gapExtend"], True, threads)[0]
+                                        sw["gapInit"], sw["gapExtend"], True, nthreads)[0]
         return time.perf_counter() - t
 
     k0 = int(np.searchsorted(csum, 2e8 * threads / 16)) + 1
@@ -126,6 +126,12 @@ def cpu_baseline(batch, target_s, gpu_scores=None):
     if gpu_scores is not None:
         res["scores_equal_gpu"] = bool(np.array_equal(np.asarray(out["s"])[:k],
                                                       np.asarray(gpu_scores)[:k]))
+    # one core on a shorter leading sample (about a tenth of the time above)
+    k1 = int(np.searchsorted(csum, g * 1e9 / threads * max(target_s / 10, 0.5))) + 1
+    k1 = min(max(k1, 1), len(cells))
+    t1 = run(k1, 1)
+    res["value_1core"] = round(float(csum[k1 - 1] / t1 / 1e9), 3)
+    res["sample_1core"] = f"first {k1} pairs ({int(csum[k1 - 1])} cells), {t1:.1f} s on 1 thread"
     return res
 
 
