@@ -59,6 +59,10 @@ def parse():
                     help="seconds of Poisson arrivals per rate (0 = skip serving)")
     ap.add_argument("--serve-unbatched", action="store_true",
                     help="also run the one-launch-per-request baseline at every rate")
+    ap.add_argument("--host-buffers", action="store_true",
+                    help="also time the host-buffer entry (ipuma_compare_local: characters in "
+                         "host memory, results back to host) on the protein batch; reported "
+                         "beside value, never as it")
     ap.add_argument("--config", choices=["protein", "skewed", "search"], default="protein",
                     help="protein: 100k BLOSUM62 pairs (the headline); skewed: 1M short + 5k "
                          "long DNA pairs, bucketed vs the plain padded batch")
@@ -392,6 +396,37 @@ def main():
         cpu = cpu_baseline(batch, args.cpu_seconds, gpu_scores=scores.cpu().numpy())
         log(f"cpu baseline: {cpu}")
 
+    host_buffers = None
+    if args.host_buffers and args.config == "protein":
+        import ipuma_amd as I
+        log("timing the host-buffer entry ...")
+        letters = np.frombuffer(b"ARNDCQEGHILKMFPSTWYVBZX*", np.uint8)
+        dec = np.zeros(256, np.uint8)
+        dec[: len(letters)] = letters
+        codes = batch["codes"]
+        abuf, aoff = I._pack([dec[codes[o:o + l]].tobytes()
+                              for o, l in zip(batch["a_off"], batch["a_len"])])
+        bbuf, boff = I._pack([dec[codes[o:o + l]].tobytes()
+                              for o, l in zip(batch["b_off"], batch["b_len"])])
+        hs, ha, hb = (np.zeros(n, np.int32) for _ in range(3))
+
+        def host_call():
+            I._check(I.lib().ipuma_compare_local(eng._h, n, abuf, aoff.ctypes.data, bbuf,
+                                                 boff.ctypes.data, hs.ctypes.data,
+                                                 ha.ctypes.data, hb.ctypes.data))
+
+        for _ in range(max(1, args.warmup)):
+            host_call()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            host_call()
+        hms = (time.perf_counter() - t0) / args.steps * 1e3
+        host_buffers = {"ms_per_step": round(hms, 3), "gcups": round(cells / hms / 1e6, 1),
+                        "scores_equal_device": bool(np.array_equal(hs, scores.cpu().numpy())),
+                        "what": "ipuma_compare_local, host characters in, host results out "
+                                "(PCIe-inclusive)"}
+        log(f"host buffers: {host_buffers}")
+
     serving = None
     if args.serve_seconds > 0:
         from ipuma_amd import loadgen
@@ -437,6 +472,7 @@ def main():
                      "kernel_ms": round(fast_avg, 3),
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,
+        "host_buffers": host_buffers,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
