@@ -425,14 +425,22 @@ int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
       copy_range(i0 + (i1 - i0) * t / nth, i0 + (i1 - i0) * (t + 1) / nth);
       done[sl].fetch_add(1, std::memory_order_release);
     };
+    // A worker that cannot start (std::system_error) leaves its parts to this
+    // thread; the started ones are always joined before returning.
     std::vector<std::thread> th;
-    for (size_t t = 1; t < nth; ++t)
-      th.emplace_back([&part, t] {
-        for (size_t sl = 0; sl < kSlices; ++sl) part(t, sl);
-      });
+    th.reserve(nth);
+    size_t started = 1;
+    try {
+      for (size_t t = 1; t < nth; ++t, ++started)
+        th.push_back(start_thread([&part, t] {
+          for (size_t sl = 0; sl < kSlices; ++sl) part(t, sl);
+        }));
+    } catch (const std::system_error &) {
+    }
     hipError_t err = hipSuccess;
     for (size_t sl = 0; sl < kSlices; ++sl) {
       part(0, sl);
+      for (size_t t = started; t < nth; ++t) part(t, sl);
       while (done[sl].load(std::memory_order_acquire) < nth) std::this_thread::yield();
       const size_t i0 = np * sl / kSlices, i1 = np * (sl + 1) / kSlices;
       if (i0 == i1 || err != hipSuccess) continue;
@@ -636,6 +644,7 @@ void executor_loop(ipuma_engine *e) {
 }  // namespace
 
 namespace ipuma {
+std::atomic<int> g_thread_start_budget{-1};
 const uint8_t *engine_code_table(const ipuma_engine *e) { return e->h_codes; }
 hipStream_t engine_stream(const ipuma_engine *e) { return e->stream; }
 int engine_check_error(ipuma_engine *e) { return check_device_error(e); }
@@ -762,6 +771,8 @@ static int ipuma_compare_local_body(ipuma_engine *e, int32_t n, const char *a_ch
   for (int32_t k = 0; k < n; ++k) {
     const int64_t la = a_offsets[k + 1] - a_offsets[k], lb = b_offsets[k + 1] - b_offsets[k];
     if (la < 0 || lb < 0) return fail(IPUMA_EINVAL, "offsets not ascending at %d", k);
+    if ((la > 0 && !a_chars) || (lb > 0 && !b_chars))
+      return fail(IPUMA_EINVAL, "character buffer is NULL");
     if (la > kMaxSeqLen || lb > kMaxSeqLen)
       return fail(IPUMA_ETOOLONG, "sequence %d longer than %d", k, kMaxSeqLen);
     ptrs[2 * k] = a_chars + a_offsets[k];
@@ -791,13 +802,15 @@ static int ipuma_compare_mn_local_body(ipuma_engine *e, int32_t nseqs, const cha
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
   std::lock_guard<std::mutex> lock_(e->mu);
   if (nseqs < 0 || ncmps < 0) return fail(IPUMA_EINVAL, "negative count");
-  if (ncmps > 0 && (!cmps || !scores || !a_range || !b_range || !offsets))
+  if (ncmps > 0 && (!cmps || !scores || !a_range || !b_range))
     return fail(IPUMA_EINVAL, "NULL buffer");
+  if (nseqs > 0 && !offsets) return fail(IPUMA_EINVAL, "offsets is NULL");
   std::vector<const char *> ptrs((size_t)nseqs);
   std::vector<int64_t> lens((size_t)nseqs);
   for (int32_t s = 0; s < nseqs; ++s) {
     lens[s] = offsets[s + 1] - offsets[s];
     if (lens[s] < 0) return fail(IPUMA_EINVAL, "offsets not ascending at %d", s);
+    if (lens[s] > 0 && !chars) return fail(IPUMA_EINVAL, "character buffer is NULL");
     if (lens[s] > kMaxSeqLen) return fail(IPUMA_ETOOLONG, "sequence %d longer than %d", s, kMaxSeqLen);
     ptrs[s] = chars + offsets[s];
   }
@@ -994,7 +1007,7 @@ static int ipuma_prepared_submit_body(ipuma_engine *e, const int32_t *inputs, in
     return fail(IPUMA_EINVAL, "bad buffers");
   std::lock_guard<std::mutex> lk(e->pj_mu);
   if (e->pj_stop) return fail(IPUMA_ECLOSED, "engine is closing");
-  if (!e->pj_thread.joinable()) e->pj_thread = std::thread(executor_loop, e);
+  if (!e->pj_thread.joinable()) e->pj_thread = start_thread([e] { executor_loop(e); });
   const int64_t id = e->pj_next++;
   PreparedJob &j = e->pj_jobs[id];
   j.in = inputs;
@@ -1042,6 +1055,8 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
   for (int32_t k = 0; k < n; ++k) {
     const int64_t la = a_offsets[k + 1] - a_offsets[k], lb = b_offsets[k + 1] - b_offsets[k];
     if (la < 0 || lb < 0) return fail(IPUMA_EINVAL, "offsets not ascending at %d", k);
+    if ((la > 0 && !a_chars) || (lb > 0 && !b_chars))
+      return fail(IPUMA_EINVAL, "character buffer is NULL");
     if (la > kMaxSeqLen || lb > kMaxSeqLen)
       return fail(IPUMA_ETOOLONG, "sequence %d longer than %d", k, kMaxSeqLen);
     ptrs[2 * k] = a_chars + a_offsets[k];

@@ -4,8 +4,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <exception>
 #include <new>
+#include <system_error>
+#include <thread>
+#include <utility>
 
 #include "../../include/ipuma_hip.h"
 
@@ -32,5 +36,21 @@ int no_throw(F &&body) {
   } catch (...) {
     return set_last_error(IPUMA_EDEVICE, "internal: unknown exception");
   }
+}
+// Host threads the library may still start; -1 (the default) is unlimited.
+// Fault injection for the tests of the thread-start failure paths: a start
+// past the limit throws std::system_error, as a failed std::thread start does.
+extern std::atomic<int> g_thread_start_budget;
+// Starts a host thread of the library (std::thread under the limit above).
+template <class F>
+std::thread start_thread(F &&f) {
+  int b = g_thread_start_budget.load(std::memory_order_relaxed);
+  while (b >= 0) {
+    if (b == 0)
+      throw std::system_error(std::make_error_code(std::errc::resource_unavailable_try_again),
+                              "thread start refused");
+    if (g_thread_start_budget.compare_exchange_weak(b, b - 1, std::memory_order_relaxed)) break;
+  }
+  return std::thread(std::forward<F>(f));
 }
 }  // namespace ipuma

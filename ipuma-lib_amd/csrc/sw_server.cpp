@@ -343,8 +343,15 @@ static int ipuma_server_create_body(const ipuma_sw_config *sw, const ipuma_algo_
       ipuma_server_destroy(s);
       return rc;
     }
-  s->dispatcher = std::thread(dispatch_loop, s);
-  s->completer = std::thread(complete_loop, s);
+  // a thread that cannot start: the one already running is stopped and joined
+  // and the server freed before the error is reported
+  try {
+    s->dispatcher = start_thread([s] { dispatch_loop(s); });
+    s->completer = start_thread([s] { complete_loop(s); });
+  } catch (const std::system_error &x) {
+    ipuma_server_destroy(s);
+    return set_last_error(IPUMA_EDEVICE, "server: thread start failed: %s", x.what());
+  }
   *out = s;
   return IPUMA_OK;
 }

@@ -274,6 +274,50 @@ class PreparedJob:
         return self.results
 
 
+def _count(x, name, given=None) -> int:
+    """Number of elements of a length array given to a device entry: the
+    tensor's, or `given` where the array is a raw pointer."""
+    if given is not None:
+        if hasattr(x, "numel") and int(x.numel()) != int(given):
+            raise ValueError(f"{name}: {int(x.numel())} elements, expected {int(given)}")
+        return int(given)
+    if hasattr(x, "numel"):
+        return int(x.numel())
+    raise TypeError(f"{name}: a raw pointer carries no length; pass a tensor or the count")
+
+
+def _dev(x, name, dtype, numel=None, min_numel=None, optional=False):
+    """Device address of an argument of a device entry point.  A CUDA tensor
+    is checked for dtype, contiguity and size; an int passes as a raw device
+    pointer.  Host arrays (numpy, CPU tensors) are refused: the kernels would
+    dereference a host address."""
+    if x is None:
+        if optional:
+            return None
+        raise ValueError(f"{name}: required")
+    if isinstance(x, bool):
+        raise TypeError(f"{name}: expected a CUDA tensor or a device pointer")
+    if isinstance(x, int):
+        return x
+    if isinstance(x, np.ndarray):
+        raise TypeError(f"{name}: a numpy array is host memory; pass a CUDA tensor")
+    if not hasattr(x, "data_ptr") or not hasattr(x, "is_cuda"):
+        raise TypeError(f"{name}: expected a CUDA tensor or a device pointer, got {type(x)}")
+    if not x.is_cuda:
+        raise TypeError(f"{name}: tensor is on {x.device}, not on the GPU")
+    want = (dtype,) if isinstance(dtype, str) else tuple(dtype)
+    got = str(x.dtype).replace("torch.", "")
+    if got not in want:
+        raise TypeError(f"{name}: dtype {got}, expected {' or '.join(want)}")
+    if not x.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
+    if numel is not None and int(x.numel()) != int(numel):
+        raise ValueError(f"{name}: {int(x.numel())} elements, expected {int(numel)}")
+    if min_numel is not None and int(x.numel()) < int(min_numel):
+        raise ValueError(f"{name}: {int(x.numel())} elements, at least {int(min_numel)} needed")
+    return x.data_ptr()
+
+
 @dataclass
 class BlockAlignmentResults:
     """ipu::batchaffine::BlockAlignmentResults (src/ipu_batch_affine.h:22-26)."""
@@ -443,33 +487,42 @@ class SWAlgorithm:
         return s, ar, br
 
     def compare_device(self, codes, codes_len, a_off, a_len, b_off, b_len, scores, a_range,
-                       b_range, stream=None):
-        """Device batch; every argument is a device pointer (int) or a tensor."""
-        def ptr(x):
-            return x if isinstance(x, int) else x.data_ptr()
-        n = int(a_len.numel()) if hasattr(a_len, "numel") else int(len(a_len))
+                       b_range, stream=None, n=None):
+        """Device batch: every argument is a CUDA tensor (uint8 codes, int64
+        offsets, int32 lengths and outputs, contiguous) or a raw device pointer
+        (int), which the caller vouches for; n is needed when a_len is one."""
+        n = _count(a_len, "a_len", n)
         _check(lib().ipuma_compare_device(
-            self._h, n, ptr(codes), int(codes_len), ptr(a_off), ptr(a_len), ptr(b_off),
-            ptr(b_len), ptr(scores), ptr(a_range), ptr(b_range), stream))
+            self._h, n, _dev(codes, "codes", "uint8", min_numel=codes_len), int(codes_len),
+            _dev(a_off, "a_off", "int64", numel=n), _dev(a_len, "a_len", "int32", numel=n),
+            _dev(b_off, "b_off", "int64", numel=n), _dev(b_len, "b_len", "int32", numel=n),
+            _dev(scores, "scores", "int32", min_numel=n),
+            _dev(a_range, "a_range", "int32", min_numel=n),
+            _dev(b_range, "b_range", "int32", min_numel=n), stream))
 
     def search_device(self, codes, codes_len, q_off, q_len, db_off, db_len, k, scores,
-                      top_idx, top_score, stream=None):
-        """Every query against every database sequence (device pointers or tensors);
-        scores may be None (only the top k per query is kept)."""
-        def ptr(x):
-            if x is None or isinstance(x, int):
-                return x
-            return x.ctypes.data if isinstance(x, np.ndarray) else x.data_ptr()
-        nq = int(q_len.numel()) if hasattr(q_len, "numel") else int(len(q_len))
-        ndb = int(db_len.numel()) if hasattr(db_len, "numel") else int(len(db_len))
+                      top_idx, top_score, stream=None, nq=None, ndb=None):
+        """Every query against every database sequence (CUDA tensors or raw
+        device pointers, as for compare_device; nq / ndb are needed when the
+        length arrays are raw pointers); scores may be None (only the top k per
+        query is kept)."""
+        nq = _count(q_len, "q_len", nq)
+        ndb = _count(db_len, "db_len", ndb)
         _check(lib().ipuma_search_device(
-            self._h, ptr(codes), int(codes_len), nq, ptr(q_off), ptr(q_len), ndb, ptr(db_off),
-            ptr(db_len), int(k), ptr(scores), ptr(top_idx), ptr(top_score), stream))
+            self._h, _dev(codes, "codes", "uint8", min_numel=codes_len), int(codes_len), nq,
+            _dev(q_off, "q_off", "int64", numel=nq), _dev(q_len, "q_len", "int32", numel=nq), ndb,
+            _dev(db_off, "db_off", "int64", numel=ndb),
+            _dev(db_len, "db_len", "int32", numel=ndb), int(k),
+            _dev(scores, "scores", "int32", min_numel=nq * ndb, optional=True),
+            _dev(top_idx, "top_idx", "int32", min_numel=nq * int(k)),
+            _dev(top_score, "top_score", "int32", min_numel=nq * int(k)), stream))
 
     def encode_device(self, chars, length, codes, stream=None):
-        def ptr(x):
-            return x if isinstance(x, int) else x.data_ptr()
-        _check(lib().ipuma_encode_device(self._h, ptr(chars), int(length), ptr(codes), stream))
+        """Characters to residue codes on the device (CUDA uint8/int8 tensors or
+        raw device pointers)."""
+        _check(lib().ipuma_encode_device(
+            self._h, _dev(chars, "chars", ("uint8", "int8"), min_numel=length), int(length),
+            _dev(codes, "codes", "uint8", min_numel=length), stream))
 
     def stats(self) -> dict:
         st = BatchStats()

@@ -81,19 +81,29 @@ def emu_lib(build_dir):
     I.lib.cache_clear()
 
 
-def test_ssw_parity_pairs_emulated(emu_lib):
+SSW_CASES = [c["name"] for c in json.load(open(os.path.join(HERE, "golden", "ssw_fixtures.json")))["cases"]]
+
+
+@pytest.mark.parametrize("name", SSW_CASES)
+def test_ssw_parity_pairs_emulated(emu_lib, name):
+    """Every SSW-frozen fixture (the reference's parity-test pairs and the seeded
+    BLOSUM62 11/1 and DNA 3/1 batches) through the emulated engine."""
     I = emu_lib
     d = json.load(open(os.path.join(HERE, "golden", "ssw_fixtures.json")))
-    case = d["cases"][0]  # 18 DNA pairs of ~100 bp, 8 lanes per pair
+    case = next(c for c in d["cases"] if c["name"] == name)
     c = case["config"]
-    eng = I.SWAlgorithm(I.SWConfig(gapInit=c["gap_init"], gapExtend=c["gap_extend"]),
-                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    names = {0: "na", 1: "blosum50", 2: "blosum62"}
+    sw = I.SWConfig(gapInit=c["gap_init"], gapExtend=c["gap_extend"], matchValue=c["match"],
+                    mismatchValue=c["mismatch"], ambiguityValue=c["ambiguity"],
+                    similarity=names[c["similarity"]],
+                    datatype="na" if c["datatype"] == 0 else "aa")
+    eng = I.SWAlgorithm(sw, I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
     s, ar, br = eng.compare([p[0] for p in case["pairs"]], [p[1] for p in case["pairs"]])
     qb, qe = I.unpack_range(ar)
     rb, re_ = I.unpack_range(br)
     for k, (score, eqb, eqe, erb, ere) in enumerate(case["expected"]):
         assert (int(s[k]), int(qb[k]), int(qe[k]), int(rb[k]), int(re_[k])) == \
-            (score, eqb, eqe, erb, ere), k
+            (score, eqb, eqe, erb, ere), (name, k)
     eng.close()
 
 
@@ -238,7 +248,10 @@ def test_search_topk_emulated(emu_lib):
         scores = np.zeros(3 * 120, np.int32)
         ti = np.zeros(3 * k, np.int32)
         ts = np.zeros(3 * k, np.int32)
-        eng.search_device(codes, len(codes), q_off, q_len, db_off, db_len, k, scores, ti, ts)
+        # device memory is host memory in the emulation: raw pointers
+        eng.search_device(codes.ctypes.data, len(codes), q_off.ctypes.data, q_len.ctypes.data,
+                          db_off.ctypes.data, db_len.ctypes.data, k, scores.ctypes.data,
+                          ti.ctypes.data, ts.ctypes.data, nq=3, ndb=120)
         assert np.array_equal(scores.reshape(3, 120), exp)
         ri, rs = _topk_reference(exp, k)
         assert np.array_equal(ti.reshape(3, k), ri), k
@@ -369,3 +382,91 @@ def test_traceback_emulated(emu_lib):
             exp = O.cigar(a, b, int(ar[k]), int(br[k]), int(s[k]), mat, gi, ge)
             assert np.array_equal(got[k], exp), (k, I.cigar_string(got[k]), O.cigar_string(exp))
         eng.close()
+
+
+def _thread_budget(I):
+    import ctypes
+    return ctypes.c_int.in_dll(I.lib(), "_ZN5ipuma21g_thread_start_budgetE")
+
+
+def test_thread_start_failures_emulated(emu_lib):
+    """Host threads that cannot start: the staging gather of a batch above
+    8 MB does the parts of the workers that never started itself (results
+    unchanged), and a server whose dispatcher or completer cannot start
+    reports IPUMA_EDEVICE with nothing left running."""
+    import oracle as O
+    I = emu_lib
+    budget = _thread_budget(I)
+    rng = np.random.default_rng(17)
+    # 9 MB of characters: 136 sequences of 65,535 residues against empty B
+    # (no DP cells), interleaved with short pairs whose results show that every
+    # slice was gathered to the right place
+    A, B = [], []
+    big = "".join(rng.choice(list("ACGT"), 65535))
+    for i in range(136):
+        A.append(big)
+        B.append("")
+        a = "".join(rng.choice(list("ACGT"), int(rng.integers(20, 80))))
+        A.append(a)
+        B.append("".join(rng.choice(list("ACGT"), int(rng.integers(5, 40)))) + a[5:50])
+    exp = O.sw_batch(A[1::2], B[1::2], gap_init=-2, gap_extend=-1, forward_only=True)
+    eng = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1,
+                                        forwardOnly=True))
+    try:
+        for cap in (0, 3, -1):
+            budget.value = cap
+            s, ar, br = eng.compare(A, B)
+            budget.value = -1
+            assert not s[0::2].any()
+            for g, e in zip((s[1::2], ar[1::2], br[1::2]), exp):
+                assert np.array_equal(g, e), cap
+        for cap in (0, 1):
+            budget.value = cap
+            with pytest.raises(I.IpumaError) as ei:
+                I.Server(I.SWConfig(), I.IPUAlgoConfig(forwardOnly=True), hbm_budget_bytes=4096)
+            budget.value = -1
+            assert ei.value.code == I.IPUMA_EDEVICE and "thread start" in str(ei.value)
+        # and the library still starts threads afterwards
+        srv = I.Server(I.SWConfig(), I.IPUAlgoConfig(forwardOnly=True), hbm_budget_bytes=4096)
+        t = srv.submit(["ACGT"], ["ACGT"])
+        assert int(srv.wait(t, 1)[0][0]) == 4
+        srv.close()
+    finally:
+        budget.value = -1
+        eng.close()
+
+
+def test_null_character_buffers_emulated(emu_lib):
+    """compare_local / compare_mn_local / traceback_local refuse a NULL
+    character buffer or offsets with IPUMA_EINVAL instead of reading them."""
+    import ctypes
+    I = emu_lib
+    L = I.lib()
+    eng = I.SWAlgorithm(I.SWConfig(), I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16,
+                                                      maxBatches=1))
+    offs = np.array([0, 4], np.int64)
+    out = [np.zeros(1, np.int32) for _ in range(3)]
+    o = [x.ctypes.data for x in out]
+    assert L.ipuma_compare_local(eng._h, 1, None, offs.ctypes.data, b"ACGT",
+                                 offs.ctypes.data, *o) == I.IPUMA_EINVAL
+    assert b"NULL" in L.ipuma_last_error()
+    assert L.ipuma_compare_local(eng._h, 1, b"ACGT", offs.ctypes.data, None,
+                                 offs.ctypes.data, *o) == I.IPUMA_EINVAL
+    cmps = np.array([0, 0], np.int32)
+    assert L.ipuma_compare_mn_local(eng._h, 1, None, offs.ctypes.data, 1, cmps.ctypes.data,
+                                    *o) == I.IPUMA_EINVAL
+    assert L.ipuma_compare_mn_local(eng._h, 1, b"ACGT", None, 0, None, None, None,
+                                    None) == I.IPUMA_EINVAL
+    cig = np.zeros(16, np.uint32)
+    coff = np.array([0, 16], np.int64)
+    clen = np.zeros(1, np.int32)
+    assert L.ipuma_traceback_local(eng._h, 1, None, offs.ctypes.data, b"ACGT", offs.ctypes.data,
+                                   *o, cig.ctypes.data, coff.ctypes.data,
+                                   clen.ctypes.data) == I.IPUMA_EINVAL
+    # empty sequences need no characters
+    zero = np.array([0, 0], np.int64)
+    assert L.ipuma_compare_local(eng._h, 1, None, zero.ctypes.data, None, zero.ctypes.data,
+                                 *o) == 0
+    assert int(out[0][0]) == 0
+    eng.close()

@@ -109,21 +109,17 @@ def test_simple_rows_as_mn(gpu):
 
 
 SSW_CASES = [c["name"] for c in load("ssw_fixtures.json")["cases"]]
-# This case hung the GPU twice.  The cause, shown in the kernel's machine
-# code, was the persistent loop of the 64-lane class: its next-group fetch
-# (one lane's atomic, read back with readfirstlane) was threaded by the
-# compiler into the first lane's result write, after which lanes 1..63 looped
-# on group 0 without lane 0.  The loop walks a fixed serpentine schedule now;
-# the case stays skipped until the fixed loop has run the other 64-lane
-# cases clean on the device.
-HUNG_CASES = {"random_blosum62_open11_ext1": "hung the GPU twice (64-lane persistent-loop "
-              "divergence, fixed since); skipped until the fix is confirmed on other cases"}
+# random_blosum62_open11_ext1 once hung the GPU.  The cause, shown in the
+# kernel's machine code, was the 64-lane class's next-group fetch: one lane's
+# atomic read back with readfirstlane, which the compiler threaded into the
+# first lane's result write, after which lanes 1..63 looped on group 0
+# without lane 0.  Every lane now executes the fetch's atomic (lane 0 adds 1,
+# the others 0) with no lane-conditional branch around it (next_group in
+# sw_kernels.hip); this case is the regression test of that fix.
 
 
 @pytest.mark.parametrize("name", SSW_CASES)
 def test_ssw_fixtures(gpu, name):
-    if name in HUNG_CASES:
-        pytest.skip(HUNG_CASES[name])
     d = load("ssw_fixtures.json")
     names = {0: "na", 1: "blosum50", 2: "blosum62"}
     case = next(c for c in d["cases"] if c["name"] == name)
