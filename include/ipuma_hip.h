@@ -137,6 +137,26 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
                          void *stream);
 
 /*
+ * ipuma_compare_device that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results as for ipuma_compare_device, forward pass only (the
+ * engine must be forward-only, else IPUMA_EINVAL).  Its device work space is a
+ * static array of about 205 MiB per GPU (held by every process that loads the
+ * library, until it exits), which bounds one call to n <= 65536 comparisons
+ * (IPUMA_EINVAL beyond) and |A| <= 4096 (a longer A is reported as
+ * IPUMA_ETOOLONG by ipuma_synchronize, and its score is written as -1).
+ * Calls on one GPU share that array: calls issued outside graph capture are
+ * ordered on the device in issue order; graph replays, and calls captured
+ * into graphs, that overlap on one GPU race.  The packed kernel runs on at
+ * most 256 blocks, the int32 kernel on 16.
+ */
+int ipuma_compare_device_noalloc(ipuma_engine *e, int32_t n, const uint8_t *codes,
+                                 int64_t codes_len, const int64_t *a_off, const int32_t *a_len,
+                                 const int64_t *b_off, const int32_t *b_len, int32_t *scores,
+                                 int32_t *a_range, int32_t *b_range, void *stream);
+
+/*
  * One-vs-many search (the database-search use of compare_local, with every
  * query against every database sequence): comparison q * ndb + d aligns
  * database sequence d (A) against query q (B); all sequences are in one

@@ -92,8 +92,10 @@ struct FastArgs {
   const int32_t *sorted;       // pair ids in bucket order (-1 = empty slot)
   const GroupDesc *groups;     // in processing order
   BucketCtl *ctl;              // group ranges per class, error bits
-  uint32_t *scratch;           // kWaveScratchWords per wave
-  uint32_t *band_scratch;      // kBandSlots * kWaveScratchWords per block
+  uint32_t *scratch;           // wave_words per wave
+  uint32_t *band_scratch;      // kBandSlots * wave_words per block
+  int64_t wave_words;          // boundary scratch of one wave or band slot, in words
+                               // (kWaveScratchWords, or less for a smaller scratch)
   int32_t *scores;
   int32_t *a_range;
   int32_t *b_range;
@@ -160,6 +162,10 @@ struct BucketArgs {
   const int32_t *b_len;
   int64_t target_slots;   // resident half-lanes x rounds: step cap = cells / (kRows * this)
   int32_t force_class;    // -1: choose per pair
+  int32_t wave_cols;      // boundary columns a wave's scratch holds: a pair's pipeline is
+                          // deep enough that mpad * 64/L <= wave_cols (kWaveScratchWords / 2)
+  int32_t max_a;          // longest A accepted (kMaxSeqLen, or less with a smaller scratch);
+                          // a longer one is an error like a length past kMaxSeqLen
   // Start pass only: the forward scores.  A pair's super-strip bin then counts
   // the rows it is expected to need before reaching its score (2 * score + 16,
   // at most |B|), so pairs that stop early share groups; nullptr otherwise.

@@ -141,6 +141,8 @@ struct ipuma_engine {
   std::vector<int32_t> h_alen, h_blen;
   ipuma_batch_stats stats{};
   BucketCtl *last_ctl = nullptr;  // device ctl of the last forward chain
+  BucketCtl *na_ctl = nullptr;    // device ctl of the allocation-free entry, when the last
+                                  // device batch was one of its calls
   int32_t last_n = 0;
   int timing = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // batch begin, fast begin/end, batch end
@@ -165,6 +167,86 @@ struct ipuma_engine {
   std::thread pj_thread;
   bool pj_stop = false;
 };
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry (ipuma_compare_device_noalloc): the
+// bucketing buffers, the boundary rows and the int32 work area of one call,
+// in one static device array per GPU, sized for at most kNaPairs comparisons
+// with |A| <= kNaMaxA.  Every call on a GPU uses that GPU's copy.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int32_t kNaPairs = 65536;  // comparisons per call
+constexpr int32_t kNaMaxA = 4096;    // longest A: a boundary row of mpad * 64/L <= 4096 columns
+constexpr int64_t kNaWaveWords = 2 * (int64_t)kNaMaxA;  // per wave and per band slot (32 KiB)
+constexpr int kNaBlocks = 256;         // persistent blocks of the packed kernel (one per CU)
+constexpr int kNaGenericBlocks = 16;   // blocks of the int32 kernel
+constexpr int64_t kNaGroups = (int64_t)kNaPairs + kRuns;
+constexpr int64_t na_align(int64_t x) { return (x + 255) / 256 * 256; }
+// byte offsets of the parts, in this order
+enum NaPart { kNaCtl, kNaHist, kNaCost, kNaBin, kNaRank, kNaOvf, kNaGbin, kNaGrank, kNaSorted,
+              kNaGtmp, kNaGroupsPart, kNaWave, kNaBand, kNaWork, kNaParts };
+constexpr int64_t na_size(int part) {
+  switch (part) {
+    case kNaCtl: return (int64_t)sizeof(BucketCtl);
+    case kNaHist: return (int64_t)kNumBins * 4;
+    case kNaCost: return (int64_t)kCostBins * 4;
+    case kNaBin: case kNaRank: case kNaOvf: return (int64_t)kNaPairs * 4;
+    case kNaGbin: case kNaGrank: return kNaGroups * 4;
+    case kNaSorted: return ((int64_t)kNaPairs + kSortedPad) * 4;
+    case kNaGtmp: case kNaGroupsPart: return kNaGroups * (int64_t)sizeof(GroupDesc);
+    case kNaWave: return (int64_t)kNaBlocks * (kFastThreads / 64) * kNaWaveWords * 4;
+    case kNaBand: return (int64_t)kNaBlocks * kBandSlots * kNaWaveWords * 4;
+    case kNaWork: return (int64_t)kNaGenericBlocks * 2 * 65536 * 4;
+    default: return 0;
+  }
+}
+constexpr int64_t na_offset(int part) {
+  int64_t o = 0;
+  for (int i = 0; i < part; ++i) o += na_align(na_size(i));
+  return o;
+}
+constexpr int64_t kNaBytes = na_offset(kNaParts);  // about 205 MiB
+static_assert(kNaBytes < ((int64_t)256 << 20), "static scratch stays below 256 MiB");
+
+// The array itself.  Host code never uses its name as an address: each GPU
+// holds its own copy, whose device address hipGetSymbolAddress gives per
+// device at engine set-up (NaDevice::ptr).
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+// Per device: the array's address there and an event that orders the calls
+// on that device (they share the array).
+constexpr int kNaMaxDevices = 64;
+struct NaDevice {
+  bool ready = false;
+  int rc = IPUMA_OK;  // set-up failure, reported by the entry point
+  std::string err;
+  uint8_t *ptr = nullptr;
+  hipEvent_t order = nullptr;
+  std::mutex mu;      // serializes the calls' host side on this device
+};
+NaDevice g_na[kNaMaxDevices];
+std::mutex g_na_setup_mu;
+
+// Once per device, with that device current (called from engine set-up).
+void na_setup(int dev) {
+  if (dev < 0 || dev >= kNaMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_na_setup_mu);
+  NaDevice &d = g_na[dev];
+  if (d.ready || d.rc != IPUMA_OK) return;
+  void *p = nullptr;
+  hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (err == hipSuccess && p != nullptr)
+    err = hipEventCreateWithFlags(&d.order, hipEventDisableTiming);
+  if (err != hipSuccess || p == nullptr) {
+    d.rc = IPUMA_EDEVICE;
+    d.err = std::string("static scratch of device ") + std::to_string(dev) + ": " +
+            (err != hipSuccess ? hipGetErrorString(err) : "no address");
+    return;
+  }
+  d.ptr = static_cast<uint8_t *>(p);
+  d.ready = true;
+}
+}  // namespace
 
 // Resident half-lanes x rounds, divided by the step cap scale: the bucketing's
 // cap on a group's steps is cells / (kRows * target_slots).
@@ -204,6 +286,7 @@ int init_engine(ipuma_engine *e) {
   HIP_TRY(hipMalloc(&e->d_wiretab, 256));
   HIP_TRY(hipMemcpy(e->d_wiretab, wiretab, 256, hipMemcpyHostToDevice));
   HIP_TRY(hipEventCreateWithFlags(&e->order_ev, hipEventDisableTiming));
+  na_setup(e->device);
   const int o = -(e->sw.gap_init + e->sw.gap_extend), x = -e->sw.gap_extend;
   e->packed_ok = o >= 0 && x >= 0 && o <= 32767 && x <= 32767;
   e->forward_only = e->algo.forward_only ? 1 : 0;
@@ -227,73 +310,109 @@ int ensure_chain(ChainBufs &c, int32_t n) {
   return IPUMA_OK;
 }
 
-// One forward chain: bucketing -> packed kernel -> int32 re-run.
-int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
-              const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
-              const int32_t *b_len, int32_t *scores, int32_t *a_range, int32_t *b_range,
-              hipStream_t st, const int32_t *target) {
-  int rc = ensure_chain(c, n);
-  if (rc) return rc;
+// Device buffers of one forward chain, wherever they live (the engine's
+// grow-only buffers, or the static array of the allocation-free entry).
+struct ChainPtrs {
+  int32_t *bin_of, *rank, *hist, *costhist, *sorted, *gbin, *grank, *ovf;
+  BucketCtl *ctl;
+  GroupDesc *gtmp, *groups;
+  uint32_t *wave, *band;  // boundary scratch: wave_words per wave / per band slot
+  int32_t *work;          // int32 kernel: 2 * 65536 ints per block
+  int64_t wave_words;
+  int32_t max_a;          // longest A accepted
+  int fast_blocks, generic_blocks;
+  int64_t target_slots;
+};
+
+// One forward chain on given buffers: bucketing -> packed kernel -> int32 re-run.
+int run_chain_on(ipuma_engine *e, const ChainPtrs &c, int32_t n, const uint8_t *codes,
+                 const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
+                 const int32_t *b_len, int32_t *scores, int32_t *a_range, int32_t *b_range,
+                 hipStream_t st, const int32_t *target, bool time_fast) {
   const bool packed = e->packed_ok && e->precision == IPUMA_PREC_AUTO;
-  BucketCtl *ctl = c.ctl.as<BucketCtl>();
+  BucketCtl *ctl = c.ctl;
   if (!packed) {
     HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t *>(ctl), sizeof(BucketCtl) / 4, 0, st));
     // every comparison through the exact int32 kernel
-    if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
     GenericArgs G{};
     G.codes = codes; G.a_off = a_off; G.a_len = a_len; G.b_off = b_off; G.b_len = b_len;
     G.table = e->d_table; G.kp = e->kp; G.list = nullptr; G.count = nullptr; G.n = n;
-    G.work = e->work.as<int32_t>(); G.gap_init = e->sw.gap_init; G.gap_extend = e->sw.gap_extend;
+    G.work = c.work; G.gap_init = e->sw.gap_init; G.gap_extend = e->sw.gap_extend;
     G.scores = scores; G.a_range = a_range; G.b_range = b_range; G.error = &ctl->error;
-    HIP_TRY(launch_generic(G, e->generic_blocks, st));
+    HIP_TRY(launch_generic(G, c.generic_blocks, st));
     return IPUMA_OK;
   }
-  const size_t scratch_bytes = (size_t)e->fast_blocks * (kFastThreads / 64) * kWaveScratchWords * 4;
-  if ((rc = e->scratch.ensure(scratch_bytes))) return rc;
-  if ((rc = e->band_scratch.ensure((size_t)e->fast_blocks * kBandSlots * kWaveScratchWords * 4)))
-    return rc;
-  if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
   {  // control block, bin and cost histograms cleared, sorted slots empty
-    int32_t *const fp[4] = {reinterpret_cast<int32_t *>(ctl), c.hist.as<int32_t>(),
-                            c.costhist.as<int32_t>(), c.sorted.as<int32_t>()};
+    int32_t *const fp[4] = {reinterpret_cast<int32_t *>(ctl), c.hist, c.costhist, c.sorted};
     const int64_t fn[4] = {(int64_t)(sizeof(BucketCtl) / 4), kNumBins, kCostBins,
                            (int64_t)n + kSortedPad};
     const int32_t fv[4] = {0, 0, 0, -1};
     HIP_TRY(launch_fill_ranges(fp, fn, fv, st));
   }
   BucketArgs B{};
-  B.n = n; B.a_len = a_len; B.b_len = b_len; B.target_slots = e->target_slots;
-  B.force_class = e->force_class; B.plain = e->batching == IPUMA_BATCH_PLAIN ? 1 : e->batching == IPUMA_BATCH_PLAIN_TILE ? 2 : 0;
-  B.max_groups = n + kRuns; B.bin_of = c.bin_of.as<int32_t>(); B.rank = c.rank.as<int32_t>();
-  B.hist = c.hist.as<int32_t>(); B.grank = c.grank.as<int32_t>();
-  B.costhist = c.costhist.as<int32_t>(); B.ctl = ctl; B.sorted = c.sorted.as<int32_t>();
-  B.gtmp = c.gtmp.as<GroupDesc>(); B.gbin = c.gbin.as<int32_t>(); B.groups = c.groups.as<GroupDesc>();
+  B.n = n; B.a_len = a_len; B.b_len = b_len; B.target_slots = c.target_slots;
+  B.force_class = e->force_class;
+  B.wave_cols = (int32_t)(c.wave_words / 2); B.max_a = c.max_a;
+  B.plain = e->batching == IPUMA_BATCH_PLAIN ? 1 : e->batching == IPUMA_BATCH_PLAIN_TILE ? 2 : 0;
+  B.max_groups = n + kRuns; B.bin_of = c.bin_of; B.rank = c.rank;
+  B.hist = c.hist; B.grank = c.grank;
+  B.costhist = c.costhist; B.ctl = ctl; B.sorted = c.sorted;
+  B.gtmp = c.gtmp; B.gbin = c.gbin; B.groups = c.groups;
   B.scores = scores; B.a_range = a_range; B.b_range = b_range; B.target = target;
   HIP_TRY(launch_bucketing(B, st));
   FastArgs A{};
   A.codes = codes; A.a_off = a_off; A.a_len = a_len; A.b_off = b_off; A.b_len = b_len;
-  A.table = e->d_table; A.sorted = c.sorted.as<int32_t>(); A.groups = c.groups.as<GroupDesc>();
-  A.ctl = ctl; A.scratch = e->scratch.as<uint32_t>(); A.band_scratch = e->band_scratch.as<uint32_t>();
+  A.table = e->d_table; A.sorted = c.sorted; A.groups = c.groups;
+  A.ctl = ctl; A.scratch = c.wave; A.band_scratch = c.band; A.wave_words = c.wave_words;
   A.scores = scores; A.a_range = a_range; A.b_range = b_range;
-  A.ovf_list = c.ovf.as<int32_t>(); A.ovf_count = &ctl->ovf_count;
+  A.ovf_list = c.ovf; A.ovf_count = &ctl->ovf_count;
   A.max_groups = n + kRuns;
   A.target = target;
   const uint32_t o = (uint32_t)(-(e->sw.gap_init + e->sw.gap_extend)) & 0xFFFFu;
   const uint32_t x = (uint32_t)(-e->sw.gap_extend) & 0xFFFFu;
   A.gap_o = o | (o << 16);
   A.gap_x = x | (x << 16);
-  const bool time_fast = e->timing && &c == &e->fwd;
   if (time_fast) HIP_TRY(hipEventRecord(e->ev[1], st));
-  HIP_TRY(launch_fast(e->kp, A, e->fast_blocks, st));
+  HIP_TRY(launch_fast(e->kp, A, c.fast_blocks, st));
   if (time_fast) HIP_TRY(hipEventRecord(e->ev[2], st));
   GenericArgs G{};
   G.codes = codes; G.a_off = a_off; G.a_len = a_len; G.b_off = b_off; G.b_len = b_len;
-  G.table = e->d_table; G.kp = e->kp; G.list = c.ovf.as<int32_t>(); G.count = &ctl->ovf_count;
-  G.n = n; G.work = e->work.as<int32_t>(); G.gap_init = e->sw.gap_init;
+  G.table = e->d_table; G.kp = e->kp; G.list = c.ovf; G.count = &ctl->ovf_count;
+  G.n = n; G.work = c.work; G.gap_init = e->sw.gap_init;
   G.gap_extend = e->sw.gap_extend; G.scores = scores; G.a_range = a_range; G.b_range = b_range;
   G.error = &ctl->error;
-  HIP_TRY(launch_generic(G, e->generic_blocks, st));
+  HIP_TRY(launch_generic(G, c.generic_blocks, st));
   return IPUMA_OK;
+}
+
+// One forward chain on the engine's grow-only buffers.
+int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
+              const int64_t *a_off, const int32_t *a_len, const int64_t *b_off,
+              const int32_t *b_len, int32_t *scores, int32_t *a_range, int32_t *b_range,
+              hipStream_t st, const int32_t *target) {
+  int rc = ensure_chain(c, n);
+  if (rc) return rc;
+  if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
+  const bool packed = e->packed_ok && e->precision == IPUMA_PREC_AUTO;
+  if (packed) {
+    const size_t scratch_bytes =
+        (size_t)e->fast_blocks * (kFastThreads / 64) * kWaveScratchWords * 4;
+    if ((rc = e->scratch.ensure(scratch_bytes))) return rc;
+    if ((rc = e->band_scratch.ensure((size_t)e->fast_blocks * kBandSlots * kWaveScratchWords * 4)))
+      return rc;
+  }
+  ChainPtrs p{};
+  p.bin_of = c.bin_of.as<int32_t>(); p.rank = c.rank.as<int32_t>(); p.hist = c.hist.as<int32_t>();
+  p.costhist = c.costhist.as<int32_t>(); p.sorted = c.sorted.as<int32_t>();
+  p.gbin = c.gbin.as<int32_t>(); p.grank = c.grank.as<int32_t>(); p.ovf = c.ovf.as<int32_t>();
+  p.ctl = c.ctl.as<BucketCtl>(); p.gtmp = c.gtmp.as<GroupDesc>(); p.groups = c.groups.as<GroupDesc>();
+  p.wave = e->scratch.as<uint32_t>(); p.band = e->band_scratch.as<uint32_t>();
+  p.work = e->work.as<int32_t>();
+  p.wave_words = kWaveScratchWords; p.max_a = kMaxSeqLen;
+  p.fast_blocks = e->fast_blocks; p.generic_blocks = e->generic_blocks;
+  p.target_slots = e->target_slots;
+  return run_chain_on(e, p, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st,
+                      target, e->timing && &c == &e->fwd);
 }
 
 // Device work on `st` after the engine's previous work on any stream (its
@@ -334,6 +453,7 @@ int compare_device_chain(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
                          int32_t *b_range, hipStream_t st) {
   if (e->timing) HIP_TRY(hipEventRecord(e->ev[0], st));
   e->ev_recorded = false;
+  e->na_ctl = nullptr;  // errors of this batch are in the chains' control blocks
   int rc = run_chain(e, e->fwd, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st,
                      nullptr);
   if (rc) return rc;
@@ -460,13 +580,22 @@ int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
 // Error bits the kernels raise in the chains' control blocks.
 int check_device_error(ipuma_engine *e) {
   int32_t err = 0;
-  for (ChainBufs *c : {&e->fwd, &e->rev}) {
-    if (!c->ctl.p) continue;
+  // the control blocks of the last device batch: the allocation-free entry's,
+  // or the forward and reverse chains'
+  BucketCtl *const chain_ctl[2] = {e->fwd.ctl.as<BucketCtl>(), e->rev.ctl.as<BucketCtl>()};
+  BucketCtl *const na_ctl[2] = {e->na_ctl, nullptr};
+  BucketCtl *const *const ctls = e->na_ctl ? na_ctl : chain_ctl;
+  for (int i = 0; i < 2; ++i) {
+    BucketCtl *const c = ctls[i];
+    if (!c) continue;
     int32_t v = 0;
-    HIP_TRY(hipMemcpy(&v, &c->ctl.as<BucketCtl>()->error, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&v, &c->error, 4, hipMemcpyDeviceToHost));
     err |= v;
   }
-  if (err & 1) return fail(IPUMA_ETOOLONG, "a sequence is longer than %d", kMaxSeqLen);
+  if (err & 1)
+    return fail(IPUMA_ETOOLONG,
+                "a sequence is longer than %d (|A| longer than %d through "
+                "ipuma_compare_device_noalloc)", kMaxSeqLen, kNaMaxA);
   if (err & 2) return fail(IPUMA_EDEVICE, "internal: an end position outside its sequence");
   if (err & 4) return fail(IPUMA_EDEVICE, "internal: a work-group descriptor out of range");
   if (err & 8) return fail(IPUMA_EDEVICE, "internal: no free band slot");
@@ -851,6 +980,76 @@ static int ipuma_compare_device_body(ipuma_engine *e, int32_t n, const uint8_t *
                              b_range, st);
 }
 
+// ipuma_compare_device without allocation: the forward chain on the static
+// scratch of the engine's device.  Calls on one device are ordered through
+// that device's event (outside graph capture); they are issued one at a time.
+static int ipuma_compare_device_noalloc_body(ipuma_engine *e, int32_t n, const uint8_t *codes,
+                                             int64_t codes_len, const int64_t *a_off,
+                                             const int32_t *a_len, const int64_t *b_off,
+                                             const int32_t *b_len, int32_t *scores,
+                                             int32_t *a_range, int32_t *b_range, void *stream) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
+  if (n < 0) return fail(IPUMA_EINVAL, "n < 0");
+  if (n > kNaPairs)
+    return fail(IPUMA_EINVAL, "%d comparisons: the allocation-free entry takes at most %d", n,
+                kNaPairs);
+  if (n > 0 && (!codes || !a_off || !a_len || !b_off || !b_len || !scores || !a_range || !b_range))
+    return fail(IPUMA_EINVAL, "NULL buffer");
+  if (!e->forward_only)
+    return fail(IPUMA_EINVAL, "the allocation-free entry computes end positions only: the "
+                              "engine must be forward-only (start positions need "
+                              "ipuma_compare_device)");
+  if (e->device < 0 || e->device >= kNaMaxDevices)
+    return fail(IPUMA_EDEVICE, "device %d has no static scratch", e->device);
+  NaDevice &d = g_na[e->device];
+  if (!d.ready) return fail(d.rc ? d.rc : IPUMA_EDEVICE, "%s", d.err.c_str());
+  (void)codes_len;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : e->stream;
+  std::lock_guard<std::mutex> dlk(d.mu);
+  int rc = begin_work(e, st);
+  if (rc) return rc;
+  const bool cap = capturing(st);
+  if (!cap) HIP_TRY(hipStreamWaitEvent(st, d.order, 0));
+  uint8_t *const base = d.ptr;
+  auto at = [&](int part) { return base + na_offset(part); };
+  ChainPtrs p{};
+  p.ctl = reinterpret_cast<BucketCtl *>(at(kNaCtl));
+  p.hist = reinterpret_cast<int32_t *>(at(kNaHist));
+  p.costhist = reinterpret_cast<int32_t *>(at(kNaCost));
+  p.bin_of = reinterpret_cast<int32_t *>(at(kNaBin));
+  p.rank = reinterpret_cast<int32_t *>(at(kNaRank));
+  p.ovf = reinterpret_cast<int32_t *>(at(kNaOvf));
+  p.gbin = reinterpret_cast<int32_t *>(at(kNaGbin));
+  p.grank = reinterpret_cast<int32_t *>(at(kNaGrank));
+  p.sorted = reinterpret_cast<int32_t *>(at(kNaSorted));
+  p.gtmp = reinterpret_cast<GroupDesc *>(at(kNaGtmp));
+  p.groups = reinterpret_cast<GroupDesc *>(at(kNaGroupsPart));
+  p.wave = reinterpret_cast<uint32_t *>(at(kNaWave));
+  p.band = reinterpret_cast<uint32_t *>(at(kNaBand));
+  p.work = reinterpret_cast<int32_t *>(at(kNaWork));
+  p.wave_words = kNaWaveWords;
+  p.max_a = kNaMaxA;
+  p.fast_blocks = std::min(e->fast_blocks, kNaBlocks);
+  p.generic_blocks = kNaGenericBlocks;
+  // the step cap follows the resident waves of this launch
+  p.target_slots = std::max<int64_t>(1, e->target_slots * p.fast_blocks / std::max(1, e->fast_blocks));
+  if (e->timing) HIP_TRY(hipEventRecord(e->ev[0], st));
+  e->ev_recorded = false;
+  rc = run_chain_on(e, p, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st,
+                    nullptr, e->timing != 0);
+  if (rc) return rc;
+  e->na_ctl = p.ctl;
+  e->last_ctl = p.ctl;
+  e->last_n = n;
+  if (e->timing) {
+    HIP_TRY(hipEventRecord(e->ev[3], st));
+    e->ev_recorded = e->packed_ok && e->precision == IPUMA_PREC_AUTO;
+  }
+  if (!cap) HIP_TRY(hipEventRecord(d.order, st));
+  return end_work(e, st);
+}
+
 static int ipuma_search_device_body(ipuma_engine *e, const uint8_t *codes, int64_t codes_len, int32_t nq,
                         const int64_t *q_off, const int32_t *q_len, int32_t ndb,
                         const int64_t *db_off, const int32_t *db_len, int32_t k,
@@ -1192,6 +1391,16 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
                          const int32_t *b_len, int32_t *scores, int32_t *a_range,
                          int32_t *b_range, void *stream) {
   return ipuma::no_throw([&] { return ipuma_compare_device_body(e, n, codes, codes_len, a_off, a_len, b_off, b_len, scores, a_range, b_range, stream); });
+}
+
+int ipuma_compare_device_noalloc(ipuma_engine *e, int32_t n, const uint8_t *codes,
+                                 int64_t codes_len, const int64_t *a_off, const int32_t *a_len,
+                                 const int64_t *b_off, const int32_t *b_len, int32_t *scores,
+                                 int32_t *a_range, int32_t *b_range, void *stream) {
+  return ipuma::no_throw([&] {
+    return ipuma_compare_device_noalloc_body(e, n, codes, codes_len, a_off, a_len, b_off, b_len,
+                                             scores, a_range, b_range, stream);
+  });
 }
 
 int ipuma_search_device(ipuma_engine *e, const uint8_t *codes, int64_t codes_len, int32_t nq,

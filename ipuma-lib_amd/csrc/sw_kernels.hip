@@ -496,7 +496,7 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
   const int nss = __builtin_amdgcn_readfirstlane(S.gd.nss);
   const int gbase = __builtin_amdgcn_readfirstlane(S.gd.base);
   uint32_t *__restrict__ bscr =
-      A.band_scratch + ((int64_t)blockIdx.x * kBandSlots + sl) * kWaveScratchWords;
+      A.band_scratch + ((int64_t)blockIdx.x * kBandSlots + sl) * A.wave_words;
 
   const int p_lo = A.sorted[gbase];
   const int p_hi = A.sorted[gbase + 1];
@@ -755,7 +755,8 @@ __device__ void band_load(const FastArgs &A, BandState &B, int cur, int lane, in
     if (tk >= nband) break;
     const GroupDesc gd = A.groups[tk];
     const bool ok = gd.cls == kNumClasses - 1 && gd.base >= 0 && gd.mpad > 0 &&
-                    gd.mpad <= kMaxSeqLen && gd.nss >= 2 && gd.nss <= kMaxStrips;
+                    gd.mpad <= kMaxSeqLen && (int64_t)gd.mpad * 2 <= A.wave_words &&
+                    gd.nss >= 2 && gd.nss <= kMaxStrips;
     if (!ok) {
       atomicOr(&A.ctl->error, 4);
       continue;
@@ -865,7 +866,7 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   if (nband > 0) band_phase<KP>(A, tab, band, lane, nband);
   const int wslot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int wave = (int)blockIdx.x * WPB + wslot;
-  uint32_t *wscr = A.scratch + (int64_t)wave * kWaveScratchWords;
+  uint32_t *wscr = A.scratch + (int64_t)wave * A.wave_words;
   SimdShare share{left, wslot, WPB, WPB};
   for (int i = 0; i < WPB; ++i)
     if (i != wslot && simd_of[i] == simd_of[wslot]) {
@@ -878,11 +879,11 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
        gi = next_group(&A.ctl->queue, nband, ng, lane)) {
     const GroupDesc gd = A.groups[gi];
     // a descriptor outside what the bucketing produces (class, scratch row
-    // mpad * 64/L <= 65536, rows <= kMaxSeqLen) is reported and skipped
+    // mpad * 64/L * 2 <= wave_words, rows <= kMaxSeqLen) is reported and skipped
     const int cls = gd.cls;
     const int lg = cls < kNumClasses - 1 ? cls : 6;
     const bool ok = cls >= 0 && cls < kNumClasses && gd.base >= 0 && gd.mpad >= 0 &&
-                    gd.mpad <= kMaxSeqLen && ((int64_t)gd.mpad << (6 - lg)) <= 65536 &&
+                    gd.mpad <= kMaxSeqLen && ((int64_t)gd.mpad << (7 - lg)) <= A.wave_words &&
                     gd.nss >= 0 && ((int64_t)(gd.nss - 1) << lg) * kRows < kMaxSeqLen &&
                     gd.last_rows >= 1 && gd.last_rows <= kRows;
     if (!ok) {
@@ -1021,7 +1022,8 @@ __global__ void k_encode(const uint8_t *chars, int64_t len, uint8_t *codes, cons
 // greedy, src/partition.cpp:197-462).
 //
 // Class: lanes per comparison L = max(L_scratch, min(L_share, L_rows)) with
-//   L_scratch = ceil(|A| / 1024)   (boundary of one wave <= 512 KiB),
+//   L_scratch = ceil(|A| * 64 / wave_cols)   (boundary of one wave <= its scratch:
+//                                    512 KiB, ceil(|A| / 1024), in the default layout),
 //   L_share   = ceil(|A||B| / share) (cells per pipeline near the fair share
 //                                    of the launch, so big pairs get more lanes),
 //   L_rows    = ceil(|B| / kRows)  (no more lanes than row strips),
@@ -1102,7 +1104,8 @@ constexpr int kClsHash = 2048;  // at most kClsThreads keys: load factor <= 1/2
 __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32_t *a_len,
                                                           const int32_t *b_len,
                                                           int64_t target_slots,
-                                                          int32_t force_class, int32_t *bin_of,
+                                                          int32_t force_class, int32_t wave_cols,
+                                                          int32_t max_a, int32_t *bin_of,
                                                           int32_t *rank, int32_t *hist,
                                                           BucketCtl *ctl, int32_t *scores,
                                                           int32_t *a_range, int32_t *b_range,
@@ -1121,7 +1124,7 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
   int bin = -1;
   if (p < n) {
     const int m = a_len[p], nb = b_len[p];
-    if (m < 0 || nb < 0 || m > kMaxSeqLen || nb > kMaxSeqLen) {
+    if (m < 0 || nb < 0 || m > max_a || m > kMaxSeqLen || nb > kMaxSeqLen) {
       atomicOr(&ctl->error, 1);
       bin_of[p] = -1;
       scores[p] = -1;
@@ -1135,13 +1138,13 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
     } else {
       int cls;
       if (force_class >= 0) {
-        // never below the class the wave scratch needs (mpad * 64 / L <= 65536)
-        cls = max(force_class, pow2_class((m + 1023) / 1024));
+        // never below the class the wave scratch needs (mpad * 64 / L <= wave_cols)
+        cls = max(force_class, pow2_class((m * 64 + wave_cols - 1) / wave_cols));
       } else {
         // The shallowest pipeline whose group takes at most `cap` steps, the
         // batch's cells spread evenly over the resident half-lanes
         // (cells / (kRows * slots)); the wave scratch sets a floor
-        // (mpad * 64/L <= 65536).  Groups of similar length then keep the
+        // (mpad * 64/L <= wave_cols).  Groups of similar length then keep the
         // longest-first queue's tail short.
         //
         // Two refinements: a comparison that spans more than one row of 16
@@ -1154,7 +1157,7 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
         const int64_t cap = max((int64_t)1, (int64_t)(ctl->total_cells /
                                                       (unsigned long long)(kRows * target_slots)));
         const int64_t capc = nb > 64 * kRows ? max((int64_t)1, cap / 3) : cap;
-        cls = max(nb > kRows ? 1 : 0, pow2_class((m + 1023) / 1024));
+        cls = max(nb > kRows ? 1 : 0, pow2_class((m * 64 + wave_cols - 1) / wave_cols));
         while (cls < kNumClasses - 1) {
           const int Lc = kClassL[cls];
           if ((int64_t)((nb + Lc * kRows - 1) / (Lc * kRows)) * (m + Lc - 1) <= capc) break;
@@ -1458,7 +1461,7 @@ __global__ void k_finish_starts(int32_t n, const int32_t *scores, const int32_t 
 // ---------------------------------------------------------------------------
 // Plain padded batches (the baselines the bucketing is measured against):
 // pairs in input order, one pair per lane unless the longest A of the batch
-// needs a deeper pipeline for the wave scratch (|A| > 1024); group g holds
+// needs a deeper pipeline for the wave scratch (mpad * 64/L <= wave_cols); group g holds
 // pairs [g*gs, (g+1)*gs).  Mode 1 pads every group to the longest A and B of
 // the whole batch, mode 2 pads each group to its own longest pair.
 // ---------------------------------------------------------------------------
@@ -1470,11 +1473,11 @@ __global__ void k_plain_setup(int32_t n, const int32_t *bin_of, int32_t *sorted)
 
 // One wave per group.
 __global__ __launch_bounds__(256) void k_plain_groups(int32_t n, int32_t per_group,
-                                                      const int32_t *sorted,
+                                                      int32_t wave_cols, const int32_t *sorted,
                                                       const int32_t *a_len,
                                                       const int32_t *b_len, BucketCtl *ctl,
                                                       GroupDesc *groups) {
-  const int cls = pow2_class((ctl->maxm + 1023) / 1024);
+  const int cls = pow2_class(((int64_t)ctl->maxm * 64 + wave_cols - 1) / wave_cols);
   const int L = kClassL[cls];
   const int gs = 128 / L;
   const int g = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -1683,12 +1686,13 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_cells, dim3(nb), dim3(256), 0, st, n, B.a_len, B.b_len, B.ctl);
   hipLaunchKernelGGL(k_classify, dim3(nblocks(std::max(n, 1), kClsThreads)), dim3(kClsThreads), 0,
                      st, n, B.a_len, B.b_len, B.target_slots, B.plain ? 0 : B.force_class,
-                     B.bin_of, B.rank, B.hist, B.ctl, B.scores, B.a_range, B.b_range,
+                     B.wave_cols, B.max_a, B.bin_of, B.rank, B.hist, B.ctl, B.scores, B.a_range, B.b_range,
                      B.plain ? nullptr : B.target);
   if (B.plain) {
     hipLaunchKernelGGL(k_plain_setup, dim3(nb), dim3(256), 0, st, n, B.bin_of, B.sorted);
     hipLaunchKernelGGL(k_plain_groups, dim3(nblocks((int64_t)n * 64 + 64, 256)), dim3(256), 0,
-                       st, n, B.plain == 2 ? 1 : 0, B.sorted, B.a_len, B.b_len, B.ctl, B.groups);
+                       st, n, B.plain == 2 ? 1 : 0, B.wave_cols, B.sorted, B.a_len, B.b_len,
+                       B.ctl, B.groups);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_scan_run, dim3(kScanRunBlocks), dim3(256), 0, st, B.hist, B.ctl);

@@ -144,6 +144,8 @@ def lib():
     L.ipuma_compare_local.argtypes = [_P, _i32, _P, _P, _P, _P, _P, _P, _P]
     L.ipuma_compare_mn_local.argtypes = [_P, _i32, _P, _P, _i32, _P, _P, _P, _P]
     L.ipuma_compare_device.argtypes = [_P, _i32, _P, _i64, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.ipuma_compare_device_noalloc.argtypes = [_P, _i32, _P, _i64, _P, _P, _P, _P, _P, _P, _P,
+                                               _P]
     L.ipuma_set_batching.argtypes = [_P, _i32]
     L.ipuma_set_precision.argtypes = [_P, _i32]
     L.ipuma_set_forward_only.argtypes = [_P, _i32]
@@ -493,6 +495,20 @@ class SWAlgorithm:
         (int), which the caller vouches for; n is needed when a_len is one."""
         n = _count(a_len, "a_len", n)
         _check(lib().ipuma_compare_device(
+            self._h, n, _dev(codes, "codes", "uint8", min_numel=codes_len), int(codes_len),
+            _dev(a_off, "a_off", "int64", numel=n), _dev(a_len, "a_len", "int32", numel=n),
+            _dev(b_off, "b_off", "int64", numel=n), _dev(b_len, "b_len", "int32", numel=n),
+            _dev(scores, "scores", "int32", min_numel=n),
+            _dev(a_range, "a_range", "int32", min_numel=n),
+            _dev(b_range, "b_range", "int32", min_numel=n), stream))
+
+    def compare_device_noalloc(self, codes, codes_len, a_off, a_len, b_off, b_len, scores,
+                               a_range, b_range, stream=None, n=None):
+        """compare_device without device allocation (ipuma_compare_device_noalloc):
+        forward-only engines, n <= 65536, |A| <= 4096, work space in a static
+        array per GPU shared by every call on it."""
+        n = _count(a_len, "a_len", n)
+        _check(lib().ipuma_compare_device_noalloc(
             self._h, n, _dev(codes, "codes", "uint8", min_numel=codes_len), int(codes_len),
             _dev(a_off, "a_off", "int64", numel=n), _dev(a_len, "a_len", "int32", numel=n),
             _dev(b_off, "b_off", "int64", numel=n), _dev(b_len, "b_len", "int32", numel=n),

@@ -4,6 +4,8 @@
 //     error bit 2 and write nothing (under ASan: no write past the buffer).
 //  2. k_fast given a corrupted group descriptor (super-strip count far past
 //     any sequence) must raise error bit 4 and skip it instead of running it.
+//  3. k_fast given a group whose boundary row exceeds the wave scratch it was
+//     given must raise error bit 4 and skip it.
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -63,9 +65,21 @@ int main() {
     A.scratch = scratch.data(); A.scores = &score; A.a_range = &ar;
     A.b_range = &br; A.ovf_list = ovf_list; A.ovf_count = &ovf;
     A.max_groups = 2; A.gap_o = 0x00010001u; A.gap_x = 0x00010001u;
+    A.wave_words = kWaveScratchWords;
     launch_fast(kp, A, 1, nullptr);
     CHECK(ctl.error & 4, "corrupted descriptor raises error bit 4");
     CHECK(score == 8, "the valid group still runs (8 matches)");
+    // 3. a group whose boundary row would not fit a smaller wave scratch:
+    // one-lane pipelines of mpad 16 need 16 * 64 * 2 = 2048 words, the real
+    // group (mpad 8) 1024
+    std::memset(&ctl, 0, sizeof(ctl));
+    ctl.ngroups = 2;
+    gd[0] = {0, 16, 2, 0, 16};
+    score = -5;
+    A.wave_words = 1024;
+    launch_fast(kp, A, 1, nullptr);
+    CHECK(ctl.error & 4, "a boundary row past the wave scratch raises error bit 4");
+    CHECK(score == 8, "the group that fits still runs");
   }
   std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
   return failures ? 1 : 0;

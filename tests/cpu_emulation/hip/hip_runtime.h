@@ -26,6 +26,8 @@
 #define __global__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (&(x))
 #define __shared__ static
 #define __HIPCC__ 1
 
@@ -249,6 +251,11 @@ inline hipError_t hipMalloc(void **p, size_t n) {
 }
 template <class T> inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((void **)p, n); }
 inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+// a __device__ variable is a host variable here: its address is the symbol's
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+  *p = const_cast<void *>(sym);
+  return hipSuccess;
+}
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, int) { std::memcpy(d, s, n); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, int, hipStream_t) {
   std::memcpy(d, s, n);

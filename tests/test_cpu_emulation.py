@@ -470,3 +470,72 @@ def test_null_character_buffers_emulated(emu_lib):
                                  *o) == 0
     assert int(out[0][0]) == 0
     eng.close()
+
+
+def test_noalloc_entry_emulated(emu_lib):
+    """The allocation-free entry on its static scratch: two calls in a row (the
+    second smaller, with long and short pairs so band groups and deeper
+    pipelines run) equal the allocating entry's results and the oracle's; its
+    limits are refused."""
+    import oracle as O
+    from ipuma_amd.workloads import arena_from_flat
+    I = emu_lib
+    rng = np.random.default_rng(29)
+
+    def batch(npairs, long_pairs):
+        seqs = []
+        for i in range(npairs):
+            m = int(rng.integers(1, 300))
+            a = rng.integers(1, 5, m).astype(np.uint8)
+            b = np.concatenate([rng.integers(1, 5, int(rng.integers(0, 200))).astype(np.uint8),
+                                a[: m // 2]])
+            seqs += [a, b]
+        for m, nb in long_pairs:  # |A| up to the 4096 limit, |B| past one super-strip
+            a = rng.integers(1, 5, m).astype(np.uint8)
+            seqs += [a, np.concatenate([a[:nb // 2], rng.integers(1, 5, nb - nb // 2).astype(np.uint8)])]
+        lens = np.array([len(x) for x in seqs], np.int64)
+        codes, offs = arena_from_flat(np.concatenate(seqs), lens)
+        return (codes, offs[0::2].copy(), lens[0::2].astype(np.int32), offs[1::2].copy(),
+                lens[1::2].astype(np.int32))
+
+    eng = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1,
+                                        forwardOnly=True))
+    mat = O.select_matrix(O.SIM_NA, 1, -1, -1)
+
+    def run(fn, b):
+        codes, a_off, a_len, b_off, b_len = b
+        n = len(a_len)
+        out = [np.full(n, -7, np.int32) for _ in range(3)]
+        fn(codes.ctypes.data, len(codes), a_off.ctypes.data, a_len.ctypes.data,
+           b_off.ctypes.data, b_len.ctypes.data, *[o.ctypes.data for o in out], n=n)
+        eng.synchronize()
+        return out
+
+    for b in (batch(150, [(4096, 1300), (2000, 2100)]), batch(40, [(700, 1100)])):
+        got = run(eng.compare_device_noalloc, b)
+        ref = run(eng.compare_device, b)
+        exp = O.sw_batch_codes(*b, mat, -2, -1, True)
+        for g, r, e in zip(got, ref, exp):
+            assert np.array_equal(g, r)
+            assert np.array_equal(g, e)
+    assert eng.stats()["pairs"] == 41
+    # limits: more than 65536 comparisons, a forward-and-start engine, |A| > 4096
+    z = np.zeros(65537, np.int32)
+    with pytest.raises(I.IpumaError) as ei:
+        eng.compare_device_noalloc(1, 4, 1, z.ctypes.data, 1, z.ctypes.data, 1, 1, 1, n=65537)
+    assert ei.value.code == I.IPUMA_EINVAL and "65536" in str(ei.value)
+    long_a = batch(0, [(4100, 20)])
+    with pytest.raises(I.IpumaError) as ei:
+        run(eng.compare_device_noalloc, long_a)
+    assert ei.value.code == I.IPUMA_ETOOLONG
+    # the error belongs to that call: later batches of either entry report their own
+    run(eng.compare_device, batch(3, []))
+    run(eng.compare_device_noalloc, batch(3, []))
+    eng.close()
+    eng2 = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
+                         I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    with pytest.raises(I.IpumaError) as ei:
+        run(eng2.compare_device_noalloc, batch(3, []))
+    assert ei.value.code == I.IPUMA_EINVAL and "forward-only" in str(ei.value)
+    eng2.close()
