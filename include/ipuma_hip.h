@@ -221,6 +221,27 @@ int ipuma_prepared_compare(ipuma_engine *e, const int32_t *inputs, int64_t input
 int ipuma_prepared_submit(ipuma_engine *e, const int32_t *inputs, int64_t input_words,
                           int32_t *results, int64_t result_words, int64_t *job);
 int ipuma_prepared_join(ipuma_engine *e, int64_t job);
+/* Times of a prepared job: the input buffer's copy to HBM and the residue
+ * gather plus device batch before the results copy (HIP events on the
+ * engine's stream), and the executor's wall time for the whole job (copies
+ * in and out included, queueing excluded). */
+typedef struct ipuma_job_times {
+  double h2d_ms;
+  double device_ms;
+  double run_ms;
+} ipuma_job_times;
+/* ipuma_prepared_join that also reports the job's times (`times` may be
+ * NULL).  Replaces what a joined Job carries: Job::h2dCycles / innerCycles
+ * (src/ipu_batch_affine.h:41-48, read from the device program's cycle
+ * counters at src/ipu_batch_affine.cpp:123-127, 345-351) and
+ * SubmittedBatch::runTick (:28-38, ticked at :81 and :127); multiplied by
+ * ipuma_device_clock_hz the times give cycles of the same meaning. */
+int ipuma_prepared_join_timed(ipuma_engine *e, int64_t job, ipuma_job_times *times);
+/* The device's peak shader clock in Hz (hipDeviceProp_t::clockRate).
+ * Replaces IPUAlgorithm::getTileClockFrequency (src/ipu_base.h:50,
+ * src/ipu_base.cpp:197-199), which converts the reference's cycle counts to
+ * seconds. */
+int ipuma_device_clock_hz(ipuma_engine *e, double *hz);
 
 /*
  * Traceback (no counterpart in the reference's IPU path; its CPU comparator,
@@ -295,6 +316,15 @@ typedef struct ipuma_batch_stats {
  * following calls (read back by ipuma_last_batch_stats). */
 int ipuma_set_timing(ipuma_engine *e, int32_t on);
 int ipuma_last_batch_stats(ipuma_engine *e, ipuma_batch_stats *out);
+
+/* Measured throughput of the VALU instructions the DP issues, on the current
+ * device: packed int16 saturating add + max (v_pk_add_i16 / v_pk_max_i16, 2
+ * ops per lane-instruction) and 32-bit add + max, in tera-ops per second
+ * (either pointer may be NULL).  Runs two short kernels (about 0.3 s).  No
+ * reference counterpart: the calibrated peak of the packed kernel's
+ * roofline, reported beside the nominal 78.6 T (256 CUs x 4 SIMDs x 16 lanes
+ * x 2.4 GHz x 2). */
+int ipuma_valu_peak(double *pk_int16_tops, double *int32_tops);
 
 /* Synchronize the engine stream. */
 int ipuma_synchronize(ipuma_engine *e);

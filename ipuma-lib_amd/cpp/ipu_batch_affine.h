@@ -20,6 +20,7 @@
 #include "ipu_config.h"
 #include "ipuma_hip.h"
 #include "partition.h"
+#include "timing.hpp"
 #include "types.h"
 
 namespace ipu {
@@ -33,13 +34,38 @@ struct BlockAlignmentResults {
   std::vector<int32_t> b_range_result;
 };
 
+// The buffers and timers of a submitted prepared batch
+// (src/ipu_batch_affine.h:28-38).  cyclesH2D / cyclesInner point at the
+// owning Job's counters.  The reference's completion channel has no
+// counterpart here (the engine's executor signals the join).
+struct SubmittedBatch {
+  int32_t *inputs_begin = nullptr;
+  int32_t *inputs_end = nullptr;
+  int32_t *results_begin = nullptr;
+  int32_t *results_end = nullptr;
+  slotToken slot = 0;
+  void *signal_done = nullptr;
+  uint64_t *cyclesH2D = nullptr;
+  uint64_t *cyclesInner = nullptr;
+  swatlib::TickTock runTick;  // the executor's run of the batch (accumulated time)
+};
+
 // A prepared batch submitted with async_submit_prepared_remote_compare
-// (src/ipu_batch_affine.h:41-48).  join() waits for its results; the input
+// (src/ipu_batch_affine.h:40-48).  join() waits for its results; the input
 // and result buffers must stay valid until then.  The caller deletes it.
+// After join: h2dCycles / innerCycles are the input copy and the device
+// batch, from HIP events, in cycles of getTileClockFrequency() (so
+// cycles / getTileClockFrequency() are seconds, as in the reference);
+// tick spans submit to join, sb.runTick holds the executor's run of it.
 struct Job {
   void join();
+  uint64_t h2dCycles = 0;
+  uint64_t innerCycles = 0;
+  swatlib::TickTock tick;
+  SubmittedBatch sb;
   ipuma_engine *engine = nullptr;
   int64_t id = 0;
+  double clock_hz = 0.0;
   bool joined = false;
 };
 
@@ -132,6 +158,11 @@ class SWAlgorithm {
   void upload(int32_t *inputs_begin, int32_t *inputs_end, slotToken slot);
   void refetch();
   std::string printTensors();
+
+  // Clock (Hz) that converts a Job's cycle counts to seconds
+  // (IPUAlgorithm::getTileClockFrequency, src/ipu_base.h:50): the device's
+  // peak shader clock.
+  double getTileClockFrequency();
 
   // The engine handle, for the device-resident path (ipuma_compare_device).
   ipuma_engine *engine() const { return engine_; }

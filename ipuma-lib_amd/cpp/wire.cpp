@@ -251,20 +251,44 @@ void SWAlgorithm::prepared_remote_compare(int32_t *ib, int32_t *ie, int32_t *rb,
 
 Job *SWAlgorithm::async_submit_prepared_remote_compare(int32_t *ib, int32_t *ie, int32_t *rb,
                                                        int32_t *re) {
-  int64_t id = 0;
-  check(ipuma_prepared_submit(engine_, ib, (int64_t)(ie - ib), rb, (int64_t)(re - rb), &id));
   Job *job = new Job();
   job->engine = engine_;
+  job->clock_hz = getTileClockFrequency();
+  job->sb.inputs_begin = ib;
+  job->sb.inputs_end = ie;
+  job->sb.results_begin = rb;
+  job->sb.results_end = re;
+  job->sb.cyclesH2D = &job->h2dCycles;
+  job->sb.cyclesInner = &job->innerCycles;
+  job->tick.tick();
+  int64_t id = 0;
+  const int rc = ipuma_prepared_submit(engine_, ib, (int64_t)(ie - ib), rb, (int64_t)(re - rb), &id);
+  if (rc != IPUMA_OK) {
+    delete job;
+    check(rc);
+  }
   job->id = id;
+  job->sb.slot = (slotToken)id;
   return job;
 }
 
 void SWAlgorithm::blocking_join_prepared_remote_compare(Job &job) { job.join(); }
 
+double SWAlgorithm::getTileClockFrequency() {
+  double hz = 0.0;
+  check(ipuma_device_clock_hz(engine_, &hz));
+  return hz;
+}
+
 void Job::join() {
   if (joined) return;
   joined = true;
-  check(ipuma_prepared_join(engine, id));
+  ipuma_job_times t{};
+  check(ipuma_prepared_join_timed(engine, id, &t));
+  tick.tock();
+  sb.runTick.add_microseconds((int64_t)(t.run_ms * 1e3 + 0.5));
+  h2dCycles = (uint64_t)(t.h2d_ms * 1e-3 * clock_hz + 0.5);
+  innerCycles = (uint64_t)(t.device_ms * 1e-3 * clock_hz + 0.5);
 }
 
 void SWAlgorithm::upload(int32_t *, int32_t *, slotToken) {

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <climits>
 #include <condition_variable>
 #include <cstdarg>
@@ -101,6 +102,7 @@ struct PreparedJob {
   int rc = 0;
   std::string err;
   bool done = false;
+  ipuma_job_times times{};  // copy in, gather + device batch (HIP events); executor wall time
 };
 
 }  // namespace
@@ -110,6 +112,7 @@ struct ipuma_engine {
   ipuma_algo_config algo{};
   int device = 0;
   int num_cu = 0;
+  double clock_hz = 0.0;  // peak shader clock
   int kp = 0;   // padded alphabet (matrix side + pad code)
   int dim = 0;  // matrix side
   hipStream_t stream = nullptr;
@@ -154,6 +157,8 @@ struct ipuma_engine {
   // prepared input buffers (the reference's wire format)
   uint8_t *d_wiretab = nullptr;  // code -> code inside the matrix
   DevBuf w_wire, w_desc, w_arena, w_res;
+  hipEvent_t w_ev[3] = {nullptr, nullptr, nullptr};  // prepared batch: begin, input copied, batch done
+  double w_h2d_ms = 0.0, w_device_ms = 0.0;          // their intervals for the last prepared batch
   // traceback
   DevBuf t_desc, t_dir, t_work, t_cig;
   std::vector<int64_t> h_src, h_dst;
@@ -262,6 +267,7 @@ int init_engine(ipuma_engine *e) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, e->device));
   e->num_cu = prop.multiProcessorCount;
+  e->clock_hz = prop.clockRate > 0 ? prop.clockRate * 1e3 : 2.4e9;  // clockRate is in kHz
   HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   std::vector<int8_t> mat(25 * 25);
   const int dim = select_matrix(e->sw.similarity, e->sw.match_value, e->sw.mismatch_value,
@@ -723,12 +729,18 @@ int prepared_impl(ipuma_engine *e, const int32_t *in, int64_t in_words, int32_t 
   int32_t *d_len = reinterpret_cast<int32_t *>(d_dst + n2);
   int32_t *d_res = e->w_res.as<int32_t>();
   hipStream_t st = e->stream;
+  if (!e->w_ev[0])
+    for (auto &ev : e->w_ev) HIP_TRY(hipEventCreate(&ev));
   if ((rc = begin_work(e, st))) return rc;
+  HIP_TRY(hipEventRecord(e->w_ev[0], st));
   if (slots > 0) {
     HIP_TRY(hipMemcpyAsync(e->w_wire.p, in, (size_t)in_words * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_src, e->h_src.data(), n2 * 8, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_dst, e->h_dst.data(), n2 * 8, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_len, e->h_len.data(), n2 * 4, hipMemcpyHostToDevice, st));
+  }
+  HIP_TRY(hipEventRecord(e->w_ev[1], st));
+  if (slots > 0) {
     HIP_TRY(launch_wire_gather((int64_t)n2, e->w_wire.as<uint8_t>(), d_src, d_dst, d_len,
                                e->d_wiretab, e->w_arena.as<uint8_t>(), st));
   }
@@ -736,10 +748,16 @@ int prepared_impl(ipuma_engine *e, const int32_t *in, int64_t in_words, int32_t 
                             d_dst + slots, d_len + slots, d_res, d_res + slots, d_res + 2 * slots,
                             st);
   if (rc) return rc;
+  HIP_TRY(hipEventRecord(e->w_ev[2], st));
   if ((rc = end_work(e, st))) return rc;
   if (res_words > 0)
     HIP_TRY(hipMemcpyAsync(res, d_res, (size_t)res_words * 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  float h2d = 0.f, dev = 0.f;
+  HIP_TRY(hipEventElapsedTime(&h2d, e->w_ev[0], e->w_ev[1]));
+  HIP_TRY(hipEventElapsedTime(&dev, e->w_ev[1], e->w_ev[2]));
+  e->w_h2d_ms = h2d;
+  e->w_device_ms = dev;
   return check_device_error(e);
 }
 
@@ -757,14 +775,20 @@ void executor_loop(ipuma_engine *e) {
       job = e->pj_jobs[id];
     }
     int rc;
+    double run_ms = 0.0;
     {
       std::lock_guard<std::mutex> g(e->mu);
+      const auto t0 = std::chrono::steady_clock::now();
       rc = prepared_impl(e, job.in, job.in_words, job.res, job.res_words, 0);
+      run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     std::lock_guard<std::mutex> lk(e->pj_mu);
     PreparedJob &j = e->pj_jobs[id];
     j.rc = rc;
     if (rc) j.err = g_last_error;
+    j.times.h2d_ms = e->w_h2d_ms;
+    j.times.device_ms = e->w_device_ms;
+    j.times.run_ms = run_ms;
     j.done = true;
     e->pj_cv.notify_all();
   }
@@ -880,6 +904,8 @@ void ipuma_engine_destroy(ipuma_engine *e) {
   if (e->d_wiretab) (void)hipFree(e->d_wiretab);
   if (e->order_ev) (void)hipEventDestroy(e->order_ev);
   for (auto &ev : e->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : e->w_ev)
     if (ev) (void)hipEventDestroy(ev);
   if (e->d_table) (void)hipFree(e->d_table);
   if (e->d_codetab) (void)hipFree(e->d_codetab);
@@ -1219,7 +1245,7 @@ static int ipuma_prepared_submit_body(ipuma_engine *e, const int32_t *inputs, in
   return IPUMA_OK;
 }
 
-static int ipuma_prepared_join_body(ipuma_engine *e, int64_t job) {
+static int ipuma_prepared_join_body(ipuma_engine *e, int64_t job, ipuma_job_times *times) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
   std::unique_lock<std::mutex> lk(e->pj_mu);
   auto it = e->pj_jobs.find(job);
@@ -1233,6 +1259,7 @@ static int ipuma_prepared_join_body(ipuma_engine *e, int64_t job) {
   const PreparedJob j = it->second;
   e->pj_jobs.erase(it);
   if (j.rc) return fail(j.rc, "%s", j.err.c_str());
+  if (times) *times = j.times;
   return IPUMA_OK;
 }
 
@@ -1421,7 +1448,17 @@ int ipuma_prepared_submit(ipuma_engine *e, const int32_t *inputs, int64_t input_
 }
 
 int ipuma_prepared_join(ipuma_engine *e, int64_t job) {
-  return ipuma::no_throw([&] { return ipuma_prepared_join_body(e, job); });
+  return ipuma::no_throw([&] { return ipuma_prepared_join_body(e, job, nullptr); });
+}
+
+int ipuma_prepared_join_timed(ipuma_engine *e, int64_t job, ipuma_job_times *times) {
+  return ipuma::no_throw([&] { return ipuma_prepared_join_body(e, job, times); });
+}
+
+int ipuma_device_clock_hz(ipuma_engine *e, double *hz) {
+  if (!e || !hz) return fail(IPUMA_EINVAL, "engine or hz is NULL");
+  *hz = e->clock_hz;
+  return IPUMA_OK;
 }
 
 int ipuma_traceback_local(ipuma_engine *e, int32_t n, const char *a_chars,

@@ -31,6 +31,7 @@ class Pool:
     """A batch's sequences as characters: concatenated A's and B's with n + 1 offsets."""
 
     def __init__(self, batch: dict, letters: np.ndarray):
+        self.batch = batch  # the encoded batch the characters come from
         self.n = len(batch["a_len"])
         self.cells = batch["a_len"].astype(np.int64) * batch["b_len"].astype(np.int64)
         self.ccells = np.concatenate([[0], np.cumsum(self.cells)])
@@ -40,7 +41,10 @@ class Pool:
             offs[1:] = np.cumsum(lens)
             src = np.repeat(batch[f"{side}_off"] - offs[:-1], lens) + np.arange(int(offs[-1]))
             chars = letters[batch["codes"][src]]
-            setattr(self, f"{side}_chars", np.ascontiguousarray(np.append(chars, 0)))
+            # one byte per residue plus a terminating 0 (np.append with a Python
+            # int would promote the array to int64)
+            setattr(self, f"{side}_chars",
+                    np.concatenate([chars, np.zeros(1, np.uint8)]).astype(np.uint8))
             setattr(self, f"{side}_offs", offs)
 
     def window(self, start: int, n: int):

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <barrier>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -231,12 +232,14 @@ enum { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDe
 enum { hipStreamNonBlocking = 1 };
 struct hipDeviceProp_t {
   int multiProcessorCount;
+  int clockRate;  // kHz
 };
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char *hipGetErrorString(hipError_t) { return "emulated error"; }
 inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
   p->multiProcessorCount = emu::sm_count;
+  p->clockRate = 2400000;
   return hipSuccess;
 }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (void *)1; return hipSuccess; }
@@ -261,9 +264,15 @@ inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, int, hipStrea
   std::memcpy(d, s, n);
   return hipSuccess;
 }
-inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (void *)1; return hipSuccess; }
+// an event holds the host time of its last record (work runs at launch)
+namespace emu {
+struct Event {
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+};
+}  // namespace emu
+inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new emu::Event(); return hipSuccess; }
 enum { hipEventDisableTiming = 2, hipHostMallocDefault = 0 };
-inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = (void *)1; return hipSuccess; }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }  // work runs at launch
 inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) {
   *p = std::malloc(n);
@@ -273,15 +282,22 @@ template <class T> inline hipError_t hipHostMalloc(T **p, size_t n, unsigned f) 
   return hipHostMalloc((void **)p, n, f);
 }
 inline hipError_t hipHostFree(void *p) { std::free(p); return hipSuccess; }
-inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t e) { delete static_cast<emu::Event *>(e); return hipSuccess; }
 enum hipStreamCaptureStatus { hipStreamCaptureStatusNone = 0, hipStreamCaptureStatusActive = 1 };
 inline hipError_t hipStreamIsCapturing(hipStream_t, hipStreamCaptureStatus *s) {
   *s = hipStreamCaptureStatusNone;
   return hipSuccess;
 }
 inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
-inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
-inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
+  static_cast<emu::Event *>(e)->t = std::chrono::steady_clock::now();
+  return hipSuccess;
+}
+inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) {
+  *ms = std::chrono::duration<float, std::milli>(static_cast<emu::Event *>(b)->t -
+                                                 static_cast<emu::Event *>(a)->t).count();
+  return hipSuccess;
+}
 template <class K>
 inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, K, int, size_t) {
   *n = 1;

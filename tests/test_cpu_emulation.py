@@ -539,3 +539,25 @@ def test_noalloc_entry_emulated(emu_lib):
         run(eng2.compare_device_noalloc, batch(3, []))
     assert ei.value.code == I.IPUMA_EINVAL and "forward-only" in str(ei.value)
     eng2.close()
+
+
+NLOHMANN = "/root/reference/extern/include"  # the reference tree's copy of nlohmann::json
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(NLOHMANN, "nlohmann", "json.hpp")),
+                    reason="nlohmann/json.hpp not available")
+def test_multidriver_shape_emulated(emu_lib, build_dir):
+    """A caller in the shape of the reference's batch driver (JSON run
+    configuration, getTileClockFrequency, prepared buffers from worker
+    threads, asynchronous jobs joined by receiver threads, batch_perf metrics
+    from the Job's cycle counts) against the emulated engine."""
+    I = emu_lib
+    exe = os.path.join(build_dir, "multidriver_emu")
+    cpp = os.path.join(ROOT, "ipuma-lib_amd", "cpp")
+    subprocess.run([CLANG, "-std=c++17", "-O1", "-pthread", "-I" + cpp,
+                    "-I" + os.path.join(ROOT, "include"), "-isystem", NLOHMANN,
+                    os.path.join(HERE, "cpp", "multidriver_test.cpp"), I.LIB_PATH,
+                    "-Wl,-rpath," + os.path.dirname(I.LIB_PATH), "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "PASSED" in r.stdout and r.stdout.count("IPUSWLOG") == 12  # 720 comparisons / 60

@@ -105,7 +105,8 @@ def test_simple_rows_as_mn(gpu):
     res = eng.get_result()
     for k, r in enumerate(d["rows"]):
         assert int(res.scores[k]) == r[6]
-        assert tuple(map(int, I.unpack_range(res.a_range_result[k:k + 1]))) == (r[2], r[3])
+        qb, qe = I.unpack_range(res.a_range_result[k:k + 1])
+        assert (int(qb[0]), int(qe[0])) == (r[2], r[3])
 
 
 SSW_CASES = [c["name"] for c in load("ssw_fixtures.json")["cases"]]
@@ -488,6 +489,58 @@ def test_cpp_facade(gpu):
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
 
 
+def test_cpp_multidriver_shape(gpu):
+    # a caller in the shape of the reference's batch driver (ipusw/ipumultidriver.hpp):
+    # JSON run configuration, getTileClockFrequency, prepared buffers, asynchronous jobs
+    # joined by receiver threads, batch_perf metrics from the Job's cycle counts
+    import subprocess
+    exe = os.path.join(os.path.dirname(I.LIB_PATH), "multidriver_test")
+    if not os.path.exists(exe):
+        pytest.skip("multidriver_test is built only where nlohmann/json.hpp is available")
+    note("C++ batch-driver-shaped caller")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert r.stdout.count("IPUSWLOG") == 12
+
+
+@pytest.mark.parametrize("kind", ["dna", "aa"])
+def test_dataset_files(gpu, kind):
+    """Dataset files as the reference's batch driver reads them
+    (ipusw/run_comparison.hpp:22-61: references from FASTA, queries one per
+    line, A = reference, B = query), through compare_local, against the
+    oracle; the C++ loader (ipu::load_data) reads the same sequences."""
+    import subprocess
+    from ipuma_amd import io as IO
+    note(f"dataset files ({kind}) through compare_local")
+    ref_path = os.path.join(GOLD, "datasets", f"{kind}_refs.fa")
+    qry_path = os.path.join(GOLD, "datasets", f"{kind}_queries.txt")
+    refs, queries = IO.load_data(ref_path), IO.load_data(qry_path)
+    assert len(refs) == len(queries) and len(refs) > 100
+    io_bin = os.path.join(os.path.dirname(I.LIB_PATH), "io_test")
+    if os.path.exists(io_bin):
+        for path, seqs in ((ref_path, refs), (qry_path, queries)):
+            out = subprocess.run([io_bin, path, "0"], capture_output=True, check=True).stdout
+            lines = out.decode("latin-1").split("\n")
+            assert int(lines[0]) == len(seqs)
+            assert [ln[1:-1] for ln in lines[1:1 + len(seqs)]] == seqs
+    if kind == "dna":
+        sw = dict(gapInit=-2, gapExtend=-1)
+        osw = dict(similarity=O.SIM_NA, datatype=O.DT_NA, gap_init=-2, gap_extend=-1)
+    else:
+        sw = dict(gapInit=-10, gapExtend=-1, similarity="blosum62", datatype="aa")
+        osw = dict(similarity=O.SIM_BLOSUM62, datatype=O.DT_AA, gap_init=-10, gap_extend=-1)
+    n = len(refs)
+    eng = I.SWAlgorithm(I.SWConfig(**sw), I.IPUAlgoConfig(
+        tilesUsed=4, maxAB=max(len(x) for x in refs), maxBatches=(n + 3) // 4))
+    eng.compare_local(refs, queries)
+    r = eng.get_result()
+    exp = O.sw_batch(refs, queries, **osw)
+    assert_same((r.scores[:n], r.a_range_result[:n], r.b_range_result[:n]), exp,
+                f"dataset {kind}")
+    eng.close()
+
+
 def test_request_server(gpu):
     note("request server: 4 threads x 6 DNA requests of 1-300 pairs vs the oracle")
     import threading
@@ -522,6 +575,115 @@ def test_request_server(gpu):
             assert np.array_equal(g, e)
     st = srv.stats()
     assert st["requests"] == len(reqs) and st["arena_used_bytes"] == 0
+    srv.close()
+
+
+@pytest.mark.parametrize("forward_only", [True, False])
+def test_serving_load(gpu, forward_only):
+    """The serving workload through the request server: DNA (plain DNA
+    generator) and protein (protein batch generator) requests of uniform
+    1..4,096 comparisons, submitted from six threads at once to one server per
+    scoring on the GPU (as the load generator runs them), each request's results
+    against the oracle.  Reference loop: ipusw/run_comparison.hpp:131-145,
+    ipusw/ipuworker.hpp:8-36."""
+    import threading
+    from ipuma_amd import loadgen as LG
+    note(f"serving load: 36 DNA/protein requests of 1-4096 pairs, 6 threads, "
+         f"forward_only={forward_only}")
+    pools = LG.make_pools(pairs=12_000, seed=31)
+    rng = np.random.default_rng(32)
+    m = 36
+    kinds = ["dna" if rng.random() < 0.5 else "protein" for _ in range(m)]
+    sizes = rng.integers(1, 4097, m)
+    sizes[:2] = (4096, 1)  # both ends of the range
+    starts = [int(rng.integers(0, pools[k][0].n - int(z) + 1)) for k, z in zip(kinds, sizes)]
+    servers = {k: I.Server(I.SWConfig(**sw), I.IPUAlgoConfig(forwardOnly=forward_only),
+                           hbm_budget_bytes=256 << 20, max_batch_pairs=1 << 16, max_wait_us=1500)
+               for k, (_, sw) in pools.items()}
+    out = [None] * m
+    errors = []
+
+    def worker(idx):
+        try:
+            tickets = []
+            for i in idx:  # submit all of this thread's requests, then wait for them
+                pool = pools[kinds[i]][0]
+                n, ap, ao, bp, bo, _ = pool.window(starts[i], int(sizes[i]))
+                tickets.append((i, servers[kinds[i]].submit_raw(n, ap, ao, bp, bo)))
+            for i, t in tickets:
+                out[i] = servers[kinds[i]].wait(t, int(sizes[i]))
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    ths = [threading.Thread(target=worker, args=(list(range(j, m, 6)),)) for j in range(6)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join(timeout=240)
+    assert not errors, errors
+    for i in range(m):
+        b = pools[kinds[i]][0].batch
+        sl = slice(starts[i], starts[i] + int(sizes[i]))
+        sw = b["sw"]
+        mat = O.select_matrix(SIM[sw["similarity"]], sw["matchValue"], sw["mismatchValue"],
+                              sw["ambiguityValue"])
+        exp = O.sw_batch_codes(b["codes"], b["a_off"][sl], b["a_len"][sl], b["b_off"][sl],
+                               b["b_len"][sl], mat, sw["gapInit"], sw["gapExtend"], forward_only)
+        assert_same(out[i][:3], exp, f"request {i} ({kinds[i]}, {int(sizes[i])} pairs)")
+    for k, srv in servers.items():
+        st = srv.stats()
+        assert st["requests"] == kinds.count(k) and st["arena_used_bytes"] == 0, (k, st)
+        srv.close()
+
+
+def test_server_admission(gpu):
+    """Admission against the HBM budget on the GPU: with blocking on, requests
+    whose total is many times the arena all complete (each waits for space);
+    with it off, a request larger than the arena is refused with IPUMA_EBUSY
+    and the server keeps serving."""
+    import threading
+    note("server admission: 8 KiB arena, blocking and refusing")
+    rng = np.random.default_rng(41)
+    reqs = []
+    for r in range(16):
+        b = W.config0(seed=200 + r, n=int(rng.integers(5, 12)))
+        A = [W.decode(b["codes"][o:o + l], False) for o, l in zip(b["a_off"], b["a_len"])]
+        B = [W.decode(b["codes"][o:o + l], False) for o, l in zip(b["b_off"], b["b_len"])]
+        reqs.append((A, B))
+    srv = I.Server(I.SWConfig(**W.DNA_SW), I.IPUAlgoConfig(forwardOnly=True),
+                   hbm_budget_bytes=8192, max_batch_pairs=64, max_wait_us=200)
+    out = [None] * len(reqs)
+    errors = []
+
+    def worker(idx):
+        try:
+            for i in idx:
+                out[i] = srv.wait(srv.submit(*reqs[i]), len(reqs[i][0]))
+        except Exception as e:
+            errors.append(e)
+
+    ths = [threading.Thread(target=worker, args=(list(range(j, len(reqs), 4)),)) for j in range(4)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join(timeout=120)
+    assert not errors, errors
+    total = sum(len(a) + len(b) for A, B in reqs for a, b in zip(A, B))
+    assert total > 4 * 8192  # the requests did not fit the arena at once
+    for (A, B), got in zip(reqs, out):
+        exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1, forward_only=True)
+        assert_same(got[:3], exp, "admission (blocking)")
+    assert srv.stats()["arena_used_bytes"] == 0
+    srv.close()
+    srv = I.Server(I.SWConfig(**W.DNA_SW), I.IPUAlgoConfig(forwardOnly=True),
+                   hbm_budget_bytes=8192, max_batch_pairs=64, block_on_full=False)
+    with pytest.raises(I.IpumaError) as ei:
+        srv.submit(["A" * 5000], ["C" * 4000])
+    assert ei.value.code == I.IPUMA_EBUSY
+    A, B = reqs[0]
+    got = srv.wait(srv.submit(A, B), len(A))
+    assert_same(got[:3], O.sw_batch(A, B, gap_init=-2, gap_extend=-1, forward_only=True),
+                "admission (after a refusal)")
     srv.close()
 
 
