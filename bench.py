@@ -487,29 +487,10 @@ def bench_search(args, peak_cal, cpu_seconds=0.0):
     return rec
 
 
-def rerun32_batch(seed=5, n=1000):
-    """DNA pairs of 2-4 kb at 95% identity, match +20 / mismatch -20, gap open 20 /
-    extend 2: optimal scores above 32,767, so the packed kernel saturates on every pair
-    and the exact int32 kernel re-runs it."""
-    import numpy as np
-    from ipuma_amd import workloads as W
-    rng = np.random.default_rng(seed)
-    pairs = []
-    for _ in range(n):
-        la = int(rng.integers(2000, 4001))
-        a = W.DNA_CODES[rng.integers(0, 4, la)]
-        bseq = a.copy()
-        sub = rng.random(la) < 0.05
-        bseq[sub] = W.DNA_CODES[rng.integers(0, 4, int(sub.sum()))]
-        pairs.append((a, bseq))
-    sw = dict(gapInit=-18, gapExtend=-2, matchValue=20, mismatchValue=-20, ambiguityValue=-20,
-              similarity="na", datatype="na")
-    return W.make_batch(pairs, "dna_int32_rerun_1000", sw, seed=seed)
-
-
 def bench_rerun32(args, peak_cal):
     """1,000 saturating 2-4 kb pairs: the exact int32 re-run on many lanes."""
-    b = rerun32_batch()
+    from ipuma_amd import workloads as W
+    b = W.saturating_pairs()
     db = DeviceBatch(b)
     steps = max(2, args.sub_steps)
     ms, kms, st = db.timed(steps, 1)

@@ -149,7 +149,7 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
  * Calls on one GPU share that array: calls issued outside graph capture are
  * ordered on the device in issue order; graph replays, and calls captured
  * into graphs, that overlap on one GPU race.  The packed kernel runs on at
- * most 256 blocks, the int32 kernel on 16.
+ * most 256 blocks, the int32 kernel on 256 waves.
  */
 int ipuma_compare_device_noalloc(ipuma_engine *e, int32_t n, const uint8_t *codes,
                                  int64_t codes_len, const int64_t *a_off, const int32_t *a_len,

@@ -124,7 +124,9 @@ struct GenericArgs {
   const int32_t *list;   // pair ids, or nullptr for 0..n-1
   const int32_t *count;  // device count of `list`, or nullptr
   int32_t n;             // pair count (bound of *count when list != nullptr)
-  int32_t *work;         // 2 * 65536 ints per wave
+  int32_t *work;         // work_stride ints per wave (its boundary row: 2 per column)
+  int64_t work_stride;   // >= 2 * longest A the kernel accepts
+  int32_t max_a;         // longest A accepted; a longer one raises error bit 0
   int32_t *error;        // set to 1 when a length exceeds kMaxSeqLen
   int32_t gap_init;
   int32_t gap_extend;

@@ -5,7 +5,7 @@
 // with no host round trip:
 //   bucketing (k_sum_cells, k_classify, k_scan_run, k_scan_runs, k_scatter,
 //   k_groups, k_scan_cost, k_order_groups)  ->  k_fast (packed int16, persistent)  ->
-//   k_generic32 (exact int32 re-run of saturated comparisons)  ->
+//   k_wave32 (exact int32 re-run of saturated comparisons)  ->
 //   [k_reverse_prefix -> same chain on the reversed prefixes -> k_finish_starts]
 // The last block only runs without forward_only, for start positions.
 #include <hip/hip_runtime.h>
@@ -53,6 +53,10 @@ int fail(int code, const char *fmt, ...) {
       return fail(IPUMA_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
                   __FILE__, __LINE__);                                                  \
   } while (0)
+
+// Work area of the int32 kernel per wave: the boundary row (H, F) of an A of
+// up to kMaxSeqLen columns.
+constexpr int64_t kWorkStride = 2 * 65536;
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -184,7 +188,8 @@ constexpr int32_t kNaPairs = 65536;  // comparisons per call
 constexpr int32_t kNaMaxA = 4096;    // longest A: a boundary row of mpad * 64/L <= 4096 columns
 constexpr int64_t kNaWaveWords = 2 * (int64_t)kNaMaxA;  // per wave and per band slot (32 KiB)
 constexpr int kNaBlocks = 256;         // persistent blocks of the packed kernel (one per CU)
-constexpr int kNaGenericBlocks = 16;   // blocks of the int32 kernel
+constexpr int kNaGenericBlocks = 256;  // waves of the int32 kernel
+constexpr int64_t kNaWorkStride = 2 * (int64_t)kNaMaxA;  // its boundary row per wave (32 KiB)
 constexpr int64_t kNaGroups = (int64_t)kNaPairs + kRuns;
 constexpr int64_t na_align(int64_t x) { return (x + 255) / 256 * 256; }
 // byte offsets of the parts, in this order
@@ -201,7 +206,7 @@ constexpr int64_t na_size(int part) {
     case kNaGtmp: case kNaGroupsPart: return kNaGroups * (int64_t)sizeof(GroupDesc);
     case kNaWave: return (int64_t)kNaBlocks * (kFastThreads / 64) * kNaWaveWords * 4;
     case kNaBand: return (int64_t)kNaBlocks * kBandSlots * kNaWaveWords * 4;
-    case kNaWork: return (int64_t)kNaGenericBlocks * 2 * 65536 * 4;
+    case kNaWork: return (int64_t)kNaGenericBlocks * kNaWorkStride * 4;
     default: return 0;
   }
 }
@@ -297,7 +302,7 @@ int init_engine(ipuma_engine *e) {
   e->packed_ok = o >= 0 && x >= 0 && o <= 32767 && x <= 32767;
   e->forward_only = e->algo.forward_only ? 1 : 0;
   e->fast_blocks = e->num_cu * fast_blocks_per_cu(e->kp);
-  e->generic_blocks = std::max(8, 2 * e->num_cu);
+  e->generic_blocks = std::max(8, 4 * e->num_cu);  // one wave each
   update_target_slots(e);
   return IPUMA_OK;
 }
@@ -323,7 +328,8 @@ struct ChainPtrs {
   BucketCtl *ctl;
   GroupDesc *gtmp, *groups;
   uint32_t *wave, *band;  // boundary scratch: wave_words per wave / per band slot
-  int32_t *work;          // int32 kernel: 2 * 65536 ints per block
+  int32_t *work;          // int32 kernel: work_stride ints per wave
+  int64_t work_stride;
   int64_t wave_words;
   int32_t max_a;          // longest A accepted
   int fast_blocks, generic_blocks;
@@ -343,7 +349,8 @@ int run_chain_on(ipuma_engine *e, const ChainPtrs &c, int32_t n, const uint8_t *
     GenericArgs G{};
     G.codes = codes; G.a_off = a_off; G.a_len = a_len; G.b_off = b_off; G.b_len = b_len;
     G.table = e->d_table; G.kp = e->kp; G.list = nullptr; G.count = nullptr; G.n = n;
-    G.work = c.work; G.gap_init = e->sw.gap_init; G.gap_extend = e->sw.gap_extend;
+    G.work = c.work; G.work_stride = c.work_stride; G.max_a = c.max_a;
+    G.gap_init = e->sw.gap_init; G.gap_extend = e->sw.gap_extend;
     G.scores = scores; G.a_range = a_range; G.b_range = b_range; G.error = &ctl->error;
     HIP_TRY(launch_generic(G, c.generic_blocks, st));
     return IPUMA_OK;
@@ -384,7 +391,8 @@ int run_chain_on(ipuma_engine *e, const ChainPtrs &c, int32_t n, const uint8_t *
   GenericArgs G{};
   G.codes = codes; G.a_off = a_off; G.a_len = a_len; G.b_off = b_off; G.b_len = b_len;
   G.table = e->d_table; G.kp = e->kp; G.list = c.ovf; G.count = &ctl->ovf_count;
-  G.n = n; G.work = c.work; G.gap_init = e->sw.gap_init;
+  G.n = n; G.work = c.work; G.work_stride = c.work_stride; G.max_a = c.max_a;
+  G.gap_init = e->sw.gap_init;
   G.gap_extend = e->sw.gap_extend; G.scores = scores; G.a_range = a_range; G.b_range = b_range;
   G.error = &ctl->error;
   HIP_TRY(launch_generic(G, c.generic_blocks, st));
@@ -398,7 +406,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
               hipStream_t st, const int32_t *target) {
   int rc = ensure_chain(c, n);
   if (rc) return rc;
-  if ((rc = e->work.ensure((size_t)e->generic_blocks * 2 * 65536 * 4))) return rc;
+  if ((rc = e->work.ensure((size_t)e->generic_blocks * kWorkStride * 4))) return rc;
   const bool packed = e->packed_ok && e->precision == IPUMA_PREC_AUTO;
   if (packed) {
     const size_t scratch_bytes =
@@ -413,7 +421,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   p.gbin = c.gbin.as<int32_t>(); p.grank = c.grank.as<int32_t>(); p.ovf = c.ovf.as<int32_t>();
   p.ctl = c.ctl.as<BucketCtl>(); p.gtmp = c.gtmp.as<GroupDesc>(); p.groups = c.groups.as<GroupDesc>();
   p.wave = e->scratch.as<uint32_t>(); p.band = e->band_scratch.as<uint32_t>();
-  p.work = e->work.as<int32_t>();
+  p.work = e->work.as<int32_t>(); p.work_stride = kWorkStride;
   p.wave_words = kWaveScratchWords; p.max_a = kMaxSeqLen;
   p.fast_blocks = e->fast_blocks; p.generic_blocks = e->generic_blocks;
   p.target_slots = e->target_slots;
@@ -1054,6 +1062,7 @@ static int ipuma_compare_device_noalloc_body(ipuma_engine *e, int32_t n, const u
   p.wave = reinterpret_cast<uint32_t *>(at(kNaWave));
   p.band = reinterpret_cast<uint32_t *>(at(kNaBand));
   p.work = reinterpret_cast<int32_t *>(at(kNaWork));
+  p.work_stride = kNaWorkStride;
   p.wave_words = kNaWaveWords;
   p.max_a = kNaMaxA;
   p.fast_blocks = std::min(e->fast_blocks, kNaBlocks);

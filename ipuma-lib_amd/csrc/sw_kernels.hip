@@ -15,9 +15,10 @@
 //                  built by the device bucketing below.  STOP: the start
 //                  pass's instance, which ends a group once every pair has
 //                  reached its forward score.
-//   k_generic32    exact int32 re-statement of the vertex loop, one comparison
-//                  per lane; re-runs int16-saturated comparisons and serves
-//                  gap settings outside the packed kernel's domain.
+//   k_wave32       exact int32 re-statement of the vertex loop, one wave per
+//                  comparison (a 64-lane pipeline over its rows); re-runs
+//                  int16-saturated comparisons and serves gap settings outside
+//                  the packed kernel's domain.
 //   k_classify ... device-side length bucketing (counting sort by lanes-per-
 //                  pair class, super-strips and |A|) and longest-first order.
 #include <hip/hip_runtime.h>
@@ -911,75 +912,162 @@ template __global__ void k_fast<26, true>(FastArgs);
 
 // ---------------------------------------------------------------------------
 // Exact int32 kernel: the vertex loop of codelet_batch_affine.cpp:62-87 with
-// the reference's own (unclamped) recurrence and bounds, one comparison per
-// lane.  C / bG live in the wave's work area: 2048 ints per lane for |A| <=
-// 1024, the whole 2 x 65536 area for one longer comparison at a time.
+// the reference's own (unclamped) recurrence, initial values and bounds, one
+// wave per comparison.  The wave is one 64-lane pipeline over the B rows, as
+// the packed kernel's whole-wave pipeline: lane k owns kW32Rows rows of each
+// super-strip of 64 * kW32Rows rows and at step t computes column j = t - k
+// of them, receiving H and F of the row above (and the column's A code) from
+// lane k - 1 through DPP; lane 0 reads the row above the super-strip from the
+// wave's work area, where lane 63 wrote it during the previous super-strip.
+// Per row i (B) and column j (A), with H(i, -1) = 0, E(i, -1) = gapInit,
+// H(-1, j) = F(-1, j) = 0:
+//   E(i,j) = max(H(i,j-1) + gI + gE, E(i,j-1) + gE)
+//   F(i,j) = max(H(i-1,j) + gI + gE, F(i-1,j) + gE)
+//   H(i,j) = max(H(i-1,j-1) + S[a_j][b_i], E(i,j), F(i,j), 0)
+// and the end is the first (i, then j) cell holding the maximum, by a strict >.
+// Re-runs int16-saturated comparisons and serves gap settings outside the
+// packed kernel's domain (and the int32 precision policy).
 // ---------------------------------------------------------------------------
-__device__ void exact_pair(const GenericArgs &G, int p, int32_t *C, int32_t *bG) {
-  const int m = G.a_len[p], n = G.b_len[p];
-  if (m > kMaxSeqLen || n > kMaxSeqLen) {
-    atomicOr(G.error, 1);
-    G.scores[p] = -1;
-    G.a_range[p] = 0;
-    G.b_range[p] = 0;
-    return;
-  }
-  if (m <= 0 || n <= 0) {
-    G.scores[p] = 0;
-    G.a_range[p] = 0;
-    G.b_range[p] = 0;
-    return;
-  }
-  const uint8_t *a = G.codes + G.a_off[p];
-  const uint8_t *b = G.codes + G.b_off[p];
-  const int gI = G.gap_init, gE = G.gap_extend, kp = G.kp;
-  for (int j = 0; j < m; ++j) {
-    C[j] = 0;
-    bG[j] = 0;
-  }
-  int s = 0, aend = 0, bend = 0;
-  for (int i = 0; i < n; ++i) {
-    int lastNoGap = 0, prevNoGap = 0, aGap = gI;
-    const int bc = min((int)b[i], kp - 1);
-    for (int j = 0; j < m; ++j) {
-      aGap = max(lastNoGap + gI + gE, aGap + gE);
-      const int bgj = max(C[j] + gI + gE, bG[j] + gE);
-      bG[j] = bgj;
-      lastNoGap = max(prevNoGap + (int)G.table[min((int)a[j], kp - 1) * kp + bc], aGap);
-      lastNoGap = max(lastNoGap, bgj);
-      lastNoGap = max(lastNoGap, 0);
-      prevNoGap = C[j];
-      C[j] = lastNoGap;
-      if (lastNoGap > s) {
-        aend = j;
-        bend = i;
-        s = lastNoGap;
+constexpr int kW32Rows = 8;
+
+__global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
+  constexpr int R = kW32Rows;
+  __shared__ int32_t tab[26 * 26];
+  const int lane = threadIdx.x;
+  const int kp = G.kp;
+  for (int i = lane; i < kp * kp; i += 64) tab[i] = G.table[i];
+  __syncthreads();
+  // boundary row of the wave: (H, F) per column
+  int32_t *const bnd = G.work + (int64_t)blockIdx.x * G.work_stride;
+  const int total = G.list ? min(*G.count, G.n) : G.n;
+  const int gI = G.gap_init, gE = G.gap_extend, gIE = gI + gE;
+  const bool first = lane == 0, last = lane == 63;
+  for (int idx = blockIdx.x; idx < total; idx += gridDim.x) {
+    const int p = G.list ? G.list[idx] : idx;
+    const int m = G.a_len[p], n = G.b_len[p];
+    if (m < 0 || n < 0 || m > G.max_a || m > kMaxSeqLen || n > kMaxSeqLen ||
+        (int64_t)2 * m > G.work_stride) {
+      if (first) {
+        atomicOr(G.error, 1);
+        G.scores[p] = -1;
+        G.a_range[p] = 0;
+        G.b_range[p] = 0;
+      }
+      continue;
+    }
+    if (m == 0 || n == 0) {
+      if (first) {
+        G.scores[p] = 0;
+        G.a_range[p] = 0;
+        G.b_range[p] = 0;
+      }
+      continue;
+    }
+    const uint8_t *a = G.codes + G.a_off[p];
+    const uint8_t *b = G.codes + G.b_off[p];
+    int best = 0, bi = 0, bj = 0;
+    const int nss = (n + 64 * R - 1) / (64 * R);
+    for (int ss = 0; ss < nss; ++ss) {
+      // lane 63's boundary stores of the previous super-strip (the last one
+      // issued in its final step) visible to every lane's loads below
+      if (ss > 0) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+      }
+      const int i0 = (ss * 64 + lane) * R;
+      const bool load_bnd = ss > 0, store_bnd = ss + 1 < nss;
+      int boff[R], H[R], E[R], rmax[R], rcol[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        boff[r] = i0 + r < n ? min((int)b[i0 + r], kp - 1) : 0;
+        H[r] = 0;   // H(i, -1)
+        E[r] = gI;  // E(i, -1)
+        rmax[r] = 0;
+        rcol[r] = 0;
+      }
+      int out_h = 0, out_f = 0, col = 0;
+      int hdiag = 0;  // H(i0 - 1, j - 1)
+      // Lane 0's inputs at step t (column t: its A code and the boundary row's
+      // H and F) come from blocks of 64 columns that every lane loads one
+      // column of, a block ahead, and lane 0 reads through a shuffle.  Lane 63
+      // overwrites column c of the boundary at step c + 63, after the load of
+      // c's block (at step c - 64 or earlier).
+      int a_cur, h_cur = 0, f_cur = 0, a_nxt, h_nxt = 0, f_nxt = 0;
+      auto load_block = [&](int c0, int &av, int &hv, int &fv) {
+        const int c = c0 + lane;
+        av = c < m ? min((int)a[c], kp - 1) * kp : 0;
+        if (load_bnd && c < m) {
+          hv = bnd[2 * c];
+          fv = bnd[2 * c + 1];
+        }
+      };
+      load_block(0, a_cur, h_cur, f_cur);
+      load_block(64, a_nxt, h_nxt, f_nxt);
+      const int T = m + 63;
+      for (int t = 0; t < T; ++t) {
+        const int j = t - lane;
+        if (t > 0 && (t & 63) == 0) {
+          a_cur = a_nxt;
+          h_cur = h_nxt;
+          f_cur = f_nxt;
+          load_block(t + 64, a_nxt, h_nxt, f_nxt);
+        }
+        const int sl = t & 63;
+        const int ac = __shfl(a_cur, sl);
+        const int bh = load_bnd ? __shfl(h_cur, sl) : 0;
+        const int bf = load_bnd ? __shfl(f_cur, sl) : 0;
+        const int up_h = (int)from_prev<kModeWave>((uint32_t)bh, (uint32_t)out_h, first);
+        const int up_f = (int)from_prev<kModeWave>((uint32_t)bf, (uint32_t)out_f, first);
+        col = (int)from_prev<kModeWave>((uint32_t)ac, (uint32_t)col, first);
+        if (j >= 0 && j < m) {
+          int hd = hdiag, hup = up_h, fup = up_f;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int sc = tab[col + boff[r]];
+            const int e = max(H[r] + gIE, E[r] + gE);
+            const int f = max(hup + gIE, fup + gE);
+            const int h = max(max(hd + sc, e), max(f, 0));
+            hd = H[r];
+            H[r] = h;
+            E[r] = e;
+            hup = h;
+            fup = f;
+            if (i0 + r < n && h > rmax[r]) {
+              rmax[r] = h;
+              rcol[r] = j;
+            }
+          }
+          out_h = hup;
+          out_f = fup;
+          if (last && store_bnd) {
+            bnd[2 * j] = hup;
+            bnd[2 * j + 1] = fup;
+          }
+          hdiag = up_h;
+        }
+      }
+      // rows in order, strict >: the lane's first maximum in B order so far
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (rmax[r] > best) {
+          best = rmax[r];
+          bi = i0 + r;
+          bj = rcol[r];
+        }
+    }
+    // lanes hold disjoint rows: (max H, min row)
+    for (int off = 1; off < 64; off <<= 1) {
+      const int oh = __shfl_xor(best, off), oi = __shfl_xor(bi, off), oj = __shfl_xor(bj, off);
+      if (oh > best || (oh == best && oi < bi)) {
+        best = oh;
+        bi = oi;
+        bj = oj;
       }
     }
-  }
-  G.scores[p] = s;
-  G.a_range[p] = (int32_t)((uint32_t)(aend & 0xFFFF) << 16);
-  G.b_range[p] = (int32_t)((uint32_t)(bend & 0xFFFF) << 16);
-}
-
-__global__ __launch_bounds__(64) void k_generic32(GenericArgs G) {
-  const int lane = threadIdx.x;
-  int32_t *work = G.work + (int64_t)blockIdx.x * (2 * 65536);
-  const int total = G.list ? min(*G.count, G.n) : G.n;
-  for (int base = blockIdx.x * 64; base < total; base += gridDim.x * 64) {
-    const int idx = base + lane;
-    const int p = idx < total ? (G.list ? G.list[idx] : idx) : -1;
-    const int m = p >= 0 ? G.a_len[p] : 0;
-    if (p >= 0 && (m <= 1024 || m > kMaxSeqLen)) exact_pair(G, p, work + lane * 2048, work + lane * 2048 + 1024);
-    // Longer comparisons of this slice, one after another in lane 0 with the
-    // whole work area; lane 0 reads their ids itself, so no cross-lane
-    // operation depends on how the lanes reconverge after the loop above.
-    if (lane == 0) {
-      for (int l = 0; l < 64 && base + l < total; ++l) {
-        const int pl = G.list ? G.list[base + l] : base + l;
-        const int ml = G.a_len[pl];
-        if (ml > 1024 && ml <= kMaxSeqLen) exact_pair(G, pl, work, work + 65536);
-      }
+    if (first) {
+      G.scores[p] = best;
+      G.a_range[p] = (int32_t)((uint32_t)(bj & 0xFFFF) << 16);
+      G.b_range[p] = (int32_t)((uint32_t)(bi & 0xFFFF) << 16);
     }
   }
 }
@@ -1754,7 +1842,7 @@ int fast_blocks_per_cu(int kp) {
 }
 
 hipError_t launch_generic(const GenericArgs &G, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_generic32, dim3(blocks), dim3(64), 0, st, G);
+  hipLaunchKernelGGL(k_wave32, dim3(blocks), dim3(64), 0, st, G);
   return hipGetLastError();
 }
 

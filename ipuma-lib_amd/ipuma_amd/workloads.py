@@ -340,6 +340,24 @@ def config4(seed: int = 4, ndb: int = 200_000, qlens=CONFIG4_QUERY_LENGTHS,
                 homologs=np.sort(hidx), seed=seed)
 
 
+def saturating_pairs(seed: int = 5, n: int = 1000, la=(2000, 4001)) -> dict:
+    """DNA pairs of 2-4 kb at 95% identity, match +20 / mismatch -20, gap open
+    20 / extend 2: optimal scores above 32,767, so the packed int16 kernel
+    saturates on every pair and the exact int32 kernel re-runs it."""
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for _ in range(n):
+        m = int(rng.integers(*la))
+        a = DNA_CODES[rng.integers(0, 4, m)]
+        b = a.copy()
+        sub = rng.random(m) < 0.05
+        b[sub] = DNA_CODES[rng.integers(0, 4, int(sub.sum()))]
+        pairs.append((a, b))
+    sw = dict(gapInit=-18, gapExtend=-2, matchValue=20, mismatchValue=-20, ambiguityValue=-20,
+              similarity="na", datatype="na")
+    return make_batch(pairs, f"dna_saturating_{n}", sw, seed=seed)
+
+
 def random_pairs(seed, n, la_range, lb_range, protein=False, related=0.5) -> dict:
     """Small random mix used by the parity tests."""
     rng = np.random.default_rng(seed)

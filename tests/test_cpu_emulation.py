@@ -561,3 +561,44 @@ def test_multidriver_shape_emulated(emu_lib, build_dir):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     assert "PASSED" in r.stdout and r.stdout.count("IPUSWLOG") == 12  # 720 comparisons / 60
+
+
+def test_int32_wave_kernel_emulated(emu_lib):
+    """The exact int32 kernel (one wave per comparison, a 64-lane pipeline of
+    8 rows per lane, the boundary row between super-strips in the wave's work
+    area): the int32 policy on pairs spanning up to three super-strips, gap
+    settings outside the packed kernel's domain (a gap that extends for free,
+    a positive gap-open value), and int16-saturated pairs re-run from the
+    packed kernel's list, against the oracle."""
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(61)
+    A, B = [], []
+    for i in range(8):
+        m = int(rng.integers(1, 200))
+        a = "".join(rng.choice(list("ACGT"), m))
+        nb = int(rng.integers(520, 1100)) if i % 4 == 0 else int(rng.integers(1, 150))
+        b = "".join(rng.choice(list("ACGT"), max(0, nb - m // 2))) + a[: m // 2]
+        A.append(a)
+        B.append(b[:nb])
+    A += ["", "ACGT", "A" * 300]
+    B += ["ACGT", "", "A" * 1100]
+    cases = [((-2, -1, 1, -1), I.PREC_INT32, (True, False)), ((0, 0, 1, -1), I.PREC_AUTO, (True,)),
+             ((3, -1, 1, -1), I.PREC_AUTO, (False,)), ((-2, -1, 127, -127), I.PREC_AUTO, (True,))]
+    for (gi, ge, ma, mi), prec, fwds in cases:
+        for fwd in fwds:
+            eng = I.SWAlgorithm(I.SWConfig(gapInit=gi, gapExtend=ge, matchValue=ma,
+                                           mismatchValue=mi, ambiguityValue=mi),
+                                I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1,
+                                                forwardOnly=fwd))
+            eng.set_precision(prec)
+            exp = O.sw_batch(A, B, match=ma, mismatch=mi, ambiguity=mi, gap_init=gi,
+                             gap_extend=ge, forward_only=fwd)
+            got = eng.compare(A, B)
+            for g, e, nm in zip(got, exp, ("scores", "a_range", "b_range")):
+                bad = np.nonzero(np.asarray(g) != np.asarray(e))[0]
+                assert bad.size == 0, ((gi, ge, ma, mi), prec, fwd, nm, bad[:5],
+                                       np.asarray(g)[bad[:5]], np.asarray(e)[bad[:5]])
+            if ma == 127:  # 300 x 127 > 32767: the long A-run pair saturates
+                assert eng.stats()["rerun_pairs"] > 0
+            eng.close()

@@ -429,6 +429,23 @@ def test_int16_saturation_rerun(gpu):
     assert_same(got, oracle_batch(batch, False), "saturation")
 
 
+@pytest.mark.parametrize("forward_only", [True, False])
+def test_int32_rerun_long_pairs(gpu, forward_only):
+    """Long saturating pairs (2-4 kb, every score above 32,767) through the
+    exact int32 kernel, one wave per pair: all re-run, 40 against the oracle,
+    with and without starts."""
+    note(f"int32 re-run of 300 saturating 2-4 kb pairs, forward_only={forward_only}")
+    batch = W.saturating_pairs(seed=9, n=300)
+    eng = engine_for(batch["sw"], forward_only)
+    got = device_run(eng, batch)
+    assert eng.stats()["rerun_pairs"] == 300
+    assert int(got[0].min()) > 32767
+    idx = np.arange(0, 300, 300 // 40)[:40]
+    sub = dict(batch, a_off=batch["a_off"][idx], a_len=batch["a_len"][idx],
+               b_off=batch["b_off"][idx], b_len=batch["b_len"][idx])
+    assert_same([g[idx] for g in got], oracle_batch(sub, forward_only), "int32 re-run")
+
+
 def test_int32_mode_matches(gpu):
     note("int32 precision mode")
     batch = W.random_pairs(77, 200, (1, 300), (1, 300), protein=True)
