@@ -18,12 +18,17 @@ constexpr int kClassL[kNumClasses] = {1, 2, 4, 8, 16, 64};
 constexpr int64_t kWaveScratchWords = 65536 * 2;
 // Longest sequence the engine accepts (positions are packed as uint16).
 constexpr int32_t kMaxSeqLen = 65535;
-// Counting-sort bins of the length bucketing: class x super-strip bin x m-bin.
-// A run is one (class, super-strip bin); groups never straddle runs, and each
-// run starts at a multiple of kRunAlign slots of the sorted pair array.
+// Counting-sort bins of the length bucketing: class x super-strip bin x
+// (rows of the last super-strip, m-bin).  A run is one (class, super-strip
+// bin); groups never straddle runs, and each run starts at a multiple of
+// kRunAlign slots of the sorted pair array.  Inside a run, pairs whose last
+// super-strip needs 8, 12 or 16 rows per lane are kept apart (a group runs
+// its last super-strip with the rows its longest B needs), then ordered by
+// |A| (kMBinsPerRows bins of each).
 constexpr int kNssBins = 64;
-constexpr int kMBins = 256;
-constexpr int kMBinWidth = 16;
+constexpr int kMBins = 512;
+constexpr int kMBinsPerRows = 170;
+static_assert(3 * kMBinsPerRows <= kMBins, "m-bins of the three last-strip row counts");
 constexpr int kRuns = kNumClasses * kNssBins;
 constexpr int kNumBins = kRuns * kMBins;
 constexpr int kRunAlign = 128;

@@ -1138,8 +1138,10 @@ __device__ __forceinline__ int nss_bin_of(int nss) {
   return min(kNssBins - 1, 32 + (int)(4.0f * __log2f((float)nss * (1.0f / 32.0f))));
 }
 
+// |A| bin inside a run: 16 columns wide up to 2048, 256 wide up to 12288,
+// then one bin (a group pads to its longest A).
 __device__ __forceinline__ int mbin_of(int m) {
-  return m <= 2048 ? (m + 15) / 16 : min(kMBins - 1, 128 + (m - 2048 + 511) / 512);
+  return m <= 2048 ? (m + 15) / 16 : min(kMBinsPerRows - 1, 128 + (m - 2048 + 255) / 256);
 }
 
 // Device-scope atomics on one address serialize across the XCDs, so the
@@ -1257,7 +1259,12 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
       // and the group's super-strips, still follow |B|)
       const int nb_key = target ? min(nb, 2 * max(target[p], 0) + 16) : nb;
       const int nss = (nb_key + L * kRows - 1) / (L * kRows);
-      bin = (cls * kNssBins + nss_bin_of(nss)) * kMBins + mbin_of(m);
+      // rows per lane the last super-strip of |B| needs (a group runs it with
+      // 8, 12 or 16, by its longest B): pairs that need fewer share groups
+      const int nss_b = (nb + L * kRows - 1) / (L * kRows);
+      const int last = (nb - (nss_b - 1) * L * kRows + L - 1) / L;
+      const int rlc = last <= 8 ? 0 : last <= 12 ? 1 : 2;
+      bin = (cls * kNssBins + nss_bin_of(nss)) * kMBins + rlc * kMBinsPerRows + mbin_of(m);
       bin_of[p] = bin;
       atomicAdd(&s_count[cls], 1);
     }
@@ -1283,20 +1290,21 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
 }
 
 // Scan of the bins in two kernels.  k_scan_run: one wave per run (class,
-// super-strip bin) of kMBins consecutive bins, four bins per lane, turns each
+// super-strip bin) of kMBins consecutive bins, eight bins per lane, turns each
 // bin's count into its offset inside the run and leaves the run's total in
 // run_start.  k_scan_runs: one thread per run gives every run its start (a
 // multiple of kRunAlign slots) and its first group.  k_scatter adds the two.
-static_assert(kMBins == 256 && kRuns % 64 == 0 && kRuns <= 1024, "scan layout");
+static_assert(kMBins == 512 && kRuns % 64 == 0 && kRuns <= 1024, "scan layout");
 constexpr int kScanRunBlocks = 16;  // x 4 waves: each wave scans every 64th run
 __global__ __launch_bounds__(256) void k_scan_run(int32_t *hist, BucketCtl *ctl) {
   const int lane = threadIdx.x & 63;
   const int nw = (int)gridDim.x * (int)(blockDim.x >> 6);
   for (int r = (int)blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6); r < kRuns;
        r += nw) {
-    int4 *h4 = reinterpret_cast<int4 *>(hist + r * kMBins) + lane;
-    const int4 c = *h4;
-    const int tot = c.x + c.y + c.z + c.w;
+    int4 *h4 = reinterpret_cast<int4 *>(hist + r * kMBins) + 2 * lane;
+    const int4 c0 = h4[0], c1 = h4[1];
+    const int t0 = c0.x + c0.y + c0.z + c0.w;
+    const int tot = t0 + c1.x + c1.y + c1.z + c1.w;
     if (__ballot(tot != 0) == 0) {  // an empty run: its offsets are already 0
       if (lane == 63) ctl->run_start[r] = 0;
       continue;
@@ -1307,12 +1315,17 @@ __global__ __launch_bounds__(256) void k_scan_run(int32_t *hist, BucketCtl *ctl)
       if (lane >= off) inc += u;
     }
     const int base = inc - tot;
-    int4 o;
-    o.x = base;
-    o.y = base + c.x;
-    o.z = o.y + c.y;
-    o.w = o.z + c.z;
-    *h4 = o;
+    int4 o0, o1;
+    o0.x = base;
+    o0.y = o0.x + c0.x;
+    o0.z = o0.y + c0.y;
+    o0.w = o0.z + c0.z;
+    o1.x = base + t0;
+    o1.y = o1.x + c1.x;
+    o1.z = o1.y + c1.y;
+    o1.w = o1.z + c1.z;
+    h4[0] = o0;
+    h4[1] = o1;
     if (lane == 63) ctl->run_start[r] = inc;
   }
 }
