@@ -91,7 +91,12 @@ def parse():
                     help="timed steps of each sub-configuration")
     ap.add_argument("--calibrate", choices=["on", "off"], default="on",
                     help="measure the packed-int16 / int32 VALU peak (ipuma_valu_peak)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    unknown = [c for c in args.subconfigs.split(",") if c and c not in ALL_SUBCONFIGS]
+    if unknown:
+        ap.error(f"--subconfigs: unknown {', '.join(unknown)} (choose from "
+                 f"{', '.join(ALL_SUBCONFIGS)})")
+    return args
 
 
 def cpu_model():

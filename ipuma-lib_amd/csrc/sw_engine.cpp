@@ -17,6 +17,7 @@
 #include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -44,6 +45,10 @@ int fail(int code, const char *fmt, ...) {
   va_end(ap);
   g_last_error = buf;
   return code;
+}
+
+double since_ms(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 #define HIP_TRY(expr)                                                                   \
@@ -153,6 +158,9 @@ struct ipuma_engine {
   int32_t last_n = 0;
   int timing = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // batch begin, fast begin/end, batch end
+  // IPUMA_LOG set: one JSON line per device batch on stderr (tag IPUSWLOG)
+  bool log_batches = false;
+  bool log_pending = false;  // a device batch ran since the last line
   bool ev_recorded = false;
   // Host calls are serialized; device work of calls on different streams is
   // ordered through order_ev (the engine's buffers are shared).
@@ -258,6 +266,10 @@ void na_setup(int dev) {
 }
 }  // namespace
 
+extern "C" {
+static void log_batch(ipuma_engine *e, const char *entry, double host_ms, int status);
+}
+
 // Resident half-lanes x rounds, divided by the step cap scale: the bucketing's
 // cap on a group's steps is cells / (kRows * target_slots).
 static void update_target_slots(ipuma_engine *e) {
@@ -301,6 +313,14 @@ int init_engine(ipuma_engine *e) {
   const int o = -(e->sw.gap_init + e->sw.gap_extend), x = -e->sw.gap_extend;
   e->packed_ok = o >= 0 && x >= 0 && o <= 32767 && x <= 32767;
   e->forward_only = e->algo.forward_only ? 1 : 0;
+  {
+    const char *lg = std::getenv("IPUMA_LOG");
+    e->log_batches = lg && *lg && std::strcmp(lg, "0") != 0;
+  }
+  if (e->log_batches) {
+    for (auto &ev : e->ev) HIP_TRY(hipEventCreate(&ev));
+    e->timing = 1;
+  }
   e->fast_blocks = e->num_cu * fast_blocks_per_cu(e->kp);
   e->generic_blocks = std::max(8, 4 * e->num_cu);  // one wave each
   update_target_slots(e);
@@ -467,6 +487,7 @@ int compare_device_chain(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
                          int32_t *b_range, hipStream_t st) {
   if (e->timing) HIP_TRY(hipEventRecord(e->ev[0], st));
   e->ev_recorded = false;
+  e->log_pending = e->log_batches;
   e->na_ctl = nullptr;  // errors of this batch are in the chains' control blocks
   int rc = run_chain(e, e->fwd, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st,
                      nullptr);
@@ -591,6 +612,8 @@ int stage_arena(ipuma_engine *e, const std::vector<const char *> &ptrs,
   return IPUMA_OK;
 }
 
+int error_from_bits(int32_t err);
+
 // Error bits the kernels raise in the chains' control blocks.
 int check_device_error(ipuma_engine *e) {
   int32_t err = 0;
@@ -606,6 +629,11 @@ int check_device_error(ipuma_engine *e) {
     HIP_TRY(hipMemcpy(&v, &c->error, 4, hipMemcpyDeviceToHost));
     err |= v;
   }
+  return error_from_bits(err);
+}
+
+// The status an entry reports for the kernels' error bits.
+int error_from_bits(int32_t err) {
   if (err & 1)
     return fail(IPUMA_ETOOLONG,
                 "a sequence is longer than %d (|A| longer than %d through "
@@ -681,6 +709,7 @@ int check_sizes(ipuma_engine *e, int32_t n, const int64_t *a_lens_or_null,
 // [scores | a ranges | b ranges], one word per slot.
 int prepared_impl(ipuma_engine *e, const int32_t *in, int64_t in_words, int32_t *res,
                   int64_t res_words, int lot) {
+  const auto t_entry = std::chrono::steady_clock::now();
   const ipuma_algo_config &c = e->algo;
   const int lots = c.transmission_programs;
   if (c.tiles_used < 0 || c.max_batches < 0 || c.bufsize < 0 || lots < 1)
@@ -766,7 +795,9 @@ int prepared_impl(ipuma_engine *e, const int32_t *in, int64_t in_words, int32_t 
   HIP_TRY(hipEventElapsedTime(&dev, e->w_ev[1], e->w_ev[2]));
   e->w_h2d_ms = h2d;
   e->w_device_ms = dev;
-  return check_device_error(e);
+  rc = check_device_error(e);
+  log_batch(e, "prepared", since_ms(t_entry), rc);
+  return rc;
 }
 
 // Executor of asynchronously submitted prepared batches, in submission order.
@@ -808,7 +839,18 @@ namespace ipuma {
 std::atomic<int> g_thread_start_budget{-1};
 const uint8_t *engine_code_table(const ipuma_engine *e) { return e->h_codes; }
 hipStream_t engine_stream(const ipuma_engine *e) { return e->stream; }
-int engine_check_error(ipuma_engine *e) { return check_device_error(e); }
+int engine_error_status(int32_t bits) { return error_from_bits(bits); }
+int engine_error_words(ipuma_engine *e, const int32_t **w0, const int32_t **w1) {
+  std::lock_guard<std::mutex> lk(e->mu);
+  if (e->na_ctl) {
+    *w0 = &e->na_ctl->error;
+    *w1 = nullptr;
+  } else {
+    *w0 = e->fwd.ctl.p ? &e->fwd.ctl.as<BucketCtl>()->error : nullptr;
+    *w1 = (!e->forward_only && e->rev.ctl.p) ? &e->rev.ctl.as<BucketCtl>()->error : nullptr;
+  }
+  return IPUMA_OK;
+}
 int set_last_error(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
@@ -926,6 +968,7 @@ static int ipuma_compare_local_body(ipuma_engine *e, int32_t n, const char *a_ch
                         int32_t *a_range, int32_t *b_range) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
   std::lock_guard<std::mutex> lock_(e->mu);
+  const auto t_entry = std::chrono::steady_clock::now();
   if (n < 0) return fail(IPUMA_EINVAL, "n < 0");
   if (n > 0 && (!a_offsets || !b_offsets || !scores || !a_range || !b_range))
     return fail(IPUMA_EINVAL, "NULL buffer");
@@ -956,7 +999,9 @@ static int ipuma_compare_local_body(ipuma_engine *e, int32_t n, const char *a_ch
     e->h_alen[k] = (int32_t)lens[2 * k];
     e->h_blen[k] = (int32_t)lens[2 * k + 1];
   }
-  return run_host_batch(e, n, total, scores, a_range, b_range);
+  rc = run_host_batch(e, n, total, scores, a_range, b_range);
+  log_batch(e, "compare_local", since_ms(t_entry), rc);
+  return rc;
 }
 
 static int ipuma_compare_mn_local_body(ipuma_engine *e, int32_t nseqs, const char *chars,
@@ -964,6 +1009,7 @@ static int ipuma_compare_mn_local_body(ipuma_engine *e, int32_t nseqs, const cha
                            int32_t *scores, int32_t *a_range, int32_t *b_range) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
   std::lock_guard<std::mutex> lock_(e->mu);
+  const auto t_entry = std::chrono::steady_clock::now();
   if (nseqs < 0 || ncmps < 0) return fail(IPUMA_EINVAL, "negative count");
   if (ncmps > 0 && (!cmps || !scores || !a_range || !b_range))
     return fail(IPUMA_EINVAL, "NULL buffer");
@@ -997,7 +1043,9 @@ static int ipuma_compare_mn_local_body(ipuma_engine *e, int32_t nseqs, const cha
     e->h_alen[k] = (int32_t)lens[ia];
     e->h_blen[k] = (int32_t)lens[ib];
   }
-  return run_host_batch(e, ncmps, total, scores, a_range, b_range);
+  rc = run_host_batch(e, ncmps, total, scores, a_range, b_range);
+  log_batch(e, "compare_mn_local", since_ms(t_entry), rc);
+  return rc;
 }
 
 static int ipuma_compare_device_body(ipuma_engine *e, int32_t n, const uint8_t *codes, int64_t codes_len,
@@ -1071,6 +1119,7 @@ static int ipuma_compare_device_noalloc_body(ipuma_engine *e, int32_t n, const u
   p.target_slots = std::max<int64_t>(1, e->target_slots * p.fast_blocks / std::max(1, e->fast_blocks));
   if (e->timing) HIP_TRY(hipEventRecord(e->ev[0], st));
   e->ev_recorded = false;
+  e->log_pending = e->log_batches;
   rc = run_chain_on(e, p, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st,
                     nullptr, e->timing != 0);
   if (rc) return rc;
@@ -1150,7 +1199,7 @@ int ipuma_set_timing(ipuma_engine *e, int32_t on) {
   std::lock_guard<std::mutex> lock_(e->mu);
   if (on && !e->ev[0])
     for (auto &ev : e->ev) HIP_TRY(hipEventCreate(&ev));
-  e->timing = on ? 1 : 0;
+  e->timing = on || e->log_batches ? 1 : 0;
   return IPUMA_OK;
 }
 
@@ -1196,9 +1245,16 @@ static int ipuma_encode_device_body(ipuma_engine *e, const char *d_chars, int64_
   return IPUMA_OK;
 }
 
+static int stats_locked(ipuma_engine *e, ipuma_batch_stats *out);
+
 static int ipuma_last_batch_stats_body(ipuma_engine *e, ipuma_batch_stats *out) {
   if (!e || !out) return fail(IPUMA_EINVAL, "bad args");
   std::lock_guard<std::mutex> lock_(e->mu);
+  return stats_locked(e, out);
+}
+
+// Statistics of the last device batch (the engine mutex is held).
+static int stats_locked(ipuma_engine *e, ipuma_batch_stats *out) {
   std::memset(out, 0, sizeof(*out));
   if (!e->last_ctl) return IPUMA_OK;
   HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1396,12 +1452,44 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
   return IPUMA_OK;
 }
 
+// One line per device batch when IPUMA_LOG is set: the batch_perf record of
+// the reference's batch driver (ipusw/ipumultidriver.hpp:103-121) with this
+// engine's counters -- comparisons, cells, padded cells, groups, int32
+// re-runs, the packed kernel's and the whole device batch's times (HIP
+// events), and the entry point's host time when it has one -- written to
+// stderr behind the tag IPUSWLOG (src/types.h:6).  The engine mutex is held.
+static void log_batch(ipuma_engine *e, const char *entry, double host_ms, int status) {
+  if (!e->log_pending) return;
+  e->log_pending = false;
+  ipuma_batch_stats st;
+  if (stats_locked(e, &st) != IPUMA_OK) return;
+  char line[768];
+  const double cells = (double)st.cells;
+  int k = std::snprintf(line, sizeof(line),
+                        "IPUSWLOG{\"tag\":\"batch_perf\",\"entry\":\"%s\",\"status\":%d,"
+                        "\"pairs\":%lld,\"cells\":%lld,\"padded_cells\":%lld,\"groups\":%lld,"
+                        "\"rerun_pairs\":%lld,\"band_groups\":%d,\"kernel_ms\":%.4f,"
+                        "\"batch_ms\":%.4f,\"gcups_inner\":%.2f,\"gcups_batch\":%.2f",
+                        entry, status, (long long)st.pairs, (long long)st.cells,
+                        (long long)st.padded_cells, (long long)st.groups,
+                        (long long)st.rerun_pairs, st.band_groups, st.fast_kernel_ms, st.batch_ms,
+                        st.fast_kernel_ms > 0 ? cells / st.fast_kernel_ms / 1e6 : 0.0,
+                        st.batch_ms > 0 ? cells / st.batch_ms / 1e6 : 0.0);
+  if (host_ms > 0 && k > 0 && k < (int)sizeof(line))
+    k += std::snprintf(line + k, sizeof(line) - k, ",\"host_ms\":%.4f,\"gcups_outer\":%.2f",
+                       host_ms, cells / host_ms / 1e6);
+  std::fprintf(stderr, "%s}\n", line);
+  std::fflush(stderr);
+}
+
 int ipuma_synchronize(ipuma_engine *e) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
   std::lock_guard<std::mutex> lock_(e->mu);
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipDeviceSynchronize());
-  return check_device_error(e);
+  const int rc = check_device_error(e);
+  log_batch(e, "device", 0.0, rc);
+  return rc;
 }
 
 // Exported entry points: the bodies above, with no exception crossing the ABI.

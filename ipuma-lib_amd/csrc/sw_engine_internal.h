@@ -18,8 +18,13 @@ namespace ipuma {
 const uint8_t *engine_code_table(const ipuma_engine *e);
 // The engine's HIP stream.
 hipStream_t engine_stream(const ipuma_engine *e);
-// Error bits the kernels of the last batch raised (after its completion).
-int engine_check_error(ipuma_engine *e);
+// Device addresses of the error words of the engine's last device batch (the
+// forward chain's and, with start positions, the reverse chain's; w1 may be
+// NULL).  A caller copies them on the batch's stream right after the batch,
+// before a later batch clears them.
+int engine_error_words(ipuma_engine *e, const int32_t **w0, const int32_t **w1);
+// The status (IPUMA_OK or an error code with its message set) for error bits.
+int engine_error_status(int32_t bits);
 // Sets the calling thread's last-error message; returns `code`.
 int set_last_error(int code, const char *fmt, ...);
 // Runs an entry point's body and turns a C++ exception escaping it (a failed

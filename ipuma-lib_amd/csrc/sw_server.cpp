@@ -57,6 +57,7 @@ struct Slot {  // descriptor / result buffers of one in-flight batch
   int64_t *h_off = nullptr;  // pinned: a_off | b_off
   int32_t *h_len = nullptr;  // pinned: a_len | b_len
   int32_t *h_res = nullptr;  // pinned: scores | a_range | b_range
+  int32_t *h_err = nullptr;  // pinned: the batch's kernel error words (forward, reverse)
   hipEvent_t done = nullptr;
   bool busy = false;
 };
@@ -110,6 +111,7 @@ void free_slot(Slot &s) {
   if (s.h_off) (void)hipHostFree(s.h_off);
   if (s.h_len) (void)hipHostFree(s.h_len);
   if (s.h_res) (void)hipHostFree(s.h_res);
+  if (s.h_err) (void)hipHostFree(s.h_err);
   if (s.done) (void)hipEventDestroy(s.done);
   s = Slot{};
 }
@@ -122,6 +124,7 @@ int alloc_slot(Slot &s, int32_t cap) {
       hipHostMalloc(&s.h_off, n * 16, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&s.h_len, n * 8, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&s.h_res, n * 12, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&s.h_err, 16, hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
     return set_last_error(IPUMA_ENOMEM, "server: batch buffers for %d pairs", cap);
   return IPUMA_OK;
@@ -225,8 +228,15 @@ void dispatch_loop(ipuma_server *s) {
       rc = ipuma_compare_device(s->e, n, s->arena, s->arena_size, sl.d_aoff, sl.d_alen, sl.d_boff,
                                 sl.d_blen, sl.d_res, sl.d_res + n, sl.d_res + 2 * (size_t)n,
                                 s->stream);
+    // the batch's error words, copied in stream order before the next batch
+    // on the stream clears them
+    const int32_t *w0 = nullptr, *w1 = nullptr;
+    sl.h_err[0] = sl.h_err[1] = 0;
+    if (rc == IPUMA_OK) engine_error_words(s->e, &w0, &w1);
     if (rc == IPUMA_OK &&
-        (hipMemcpyAsync(sl.h_res, sl.d_res, (size_t)n * 12, hipMemcpyDeviceToHost, s->stream) !=
+        ((w0 && hipMemcpyAsync(sl.h_err, w0, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess) ||
+         (w1 && hipMemcpyAsync(sl.h_err + 1, w1, 4, hipMemcpyDeviceToHost, s->stream) != hipSuccess) ||
+         hipMemcpyAsync(sl.h_res, sl.d_res, (size_t)n * 12, hipMemcpyDeviceToHost, s->stream) !=
              hipSuccess ||
          hipEventRecord(sl.done, s->stream) != hipSuccess))
       rc = IPUMA_EDEVICE;
@@ -263,7 +273,7 @@ void complete_loop(ipuma_server *s) {
     Slot &sl = s->slots[b->slot];
     lk.unlock();
     int rc = hipEventSynchronize(sl.done) == hipSuccess ? IPUMA_OK : IPUMA_EDEVICE;
-    if (rc == IPUMA_OK) rc = engine_check_error(s->e);
+    if (rc == IPUMA_OK) rc = engine_error_status(sl.h_err[0] | sl.h_err[1]);
     const auto now = Clock::now();
     int64_t k = 0;
     for (Request *r : b->reqs) {

@@ -194,11 +194,11 @@ int main() {
     expect(ej == "blosum50", "json Similarity");
   }
 
-  // 40 copies of the golden rows, 720 comparisons, cut into batches of at
+  // 20 copies of the golden rows, 360 comparisons, cut into batches of at
   // most 60 comparisons (the limits of ipusw/run_comparison.hpp:93-95)
   ipu::RawSequences A, B;
   std::vector<int> row_of;
-  for (int rep = 0; rep < 40; ++rep)
+  for (int rep = 0; rep < 20; ++rep)
     for (int r = 0; r < kNumRows; ++r) {
       const int k = (r * 7 + rep) % kNumRows;
       A.push_back(kRows[k].query);

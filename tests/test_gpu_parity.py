@@ -464,6 +464,32 @@ def test_empty_and_tiny(gpu):
     assert_same((s, ar, br), exp, "empty")
 
 
+def test_batch_log_lines(gpu, capfd, monkeypatch):
+    # IPUMA_LOG at engine creation: one IPUSWLOG batch_perf JSON line per
+    # device batch, with the batch's counts and its HIP-event times
+    note("IPUMA_LOG batch_perf lines")
+    batch = W.random_pairs(57, 2000, (50, 400), (50, 400))
+    cells = batch["cells"]
+    monkeypatch.setenv("IPUMA_LOG", "1")
+    eng = engine_for(W.DNA_SW, True)
+    monkeypatch.delenv("IPUMA_LOG")
+    capfd.readouterr()
+    got = device_run(eng, batch)
+    host = eng.compare(["ACGT" * 30], ["ACGT" * 20])
+    eng.close()
+    assert_same(got, oracle_batch(batch, True), "logged batch")
+    assert int(host[0][0]) == 80
+    lines = [json.loads(ln[len("IPUSWLOG"):]) for ln in capfd.readouterr().err.splitlines()
+             if ln.startswith("IPUSWLOG")]
+    assert [ln["entry"] for ln in lines] == ["device", "compare_local"]
+    dev, loc = lines
+    assert dev["tag"] == "batch_perf" and dev["status"] == 0
+    assert dev["pairs"] == 2000 and dev["cells"] == cells
+    assert dev["padded_cells"] >= cells and dev["groups"] > 0
+    assert 0 < dev["kernel_ms"] <= dev["batch_ms"] and dev["gcups_inner"] > 0
+    assert loc["pairs"] == 1 and loc["cells"] == 120 * 80 and loc["host_ms"] > 0
+
+
 def test_gap_settings_outside_packed_domain(gpu):
     # a positive gap extension is outside the packed kernel; the exact int32
     # kernel takes over and still matches the reference recurrence
@@ -518,7 +544,7 @@ def test_cpp_multidriver_shape(gpu):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     print(r.stdout[-3000:])
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
-    assert r.stdout.count("IPUSWLOG") == 12
+    assert r.stdout.count("IPUSWLOG") == 6  # 360 comparisons / 60
 
 
 @pytest.mark.parametrize("kind", ["dna", "aa"])
