@@ -514,8 +514,8 @@ def bench_rerun32(args, peak_cal):
            "rerun_pairs": st["rerun_pairs"], "min_score": int(got[0].min()),
            "parity": {"oracle_60_pairs": parity},
            "roofline": roofline(cells, st["batch_ms"] - st["fast_kernel_ms"],
-                                "k_generic32 (device batch minus the packed kernel, HIP events)",
-                                None, precision="int32")}
+                                "k_wave32 (device batch minus the packed kernel, HIP events)",
+                                getattr(args, "pk32", None), precision="int32")}
     log(f"int32 re-run: {rec['ms_per_step']} ms/step, {rec['gcups']} GCUPS, "
         f"{st['rerun_pairs']} re-run pairs, parity {rec['parity']}")
     return rec
@@ -558,6 +558,7 @@ def main():
     if peak:
         log(f"valu peak: {peak}")
     pk16 = peak["pk_int16_tops"] if peak else None
+    args.pk32 = peak["int32_tops"] if peak else None
 
     if args.config != "protein":
         fn = {"skewed": bench_skewed, "search": bench_search, "dna1024": bench_dna1024,
