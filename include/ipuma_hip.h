@@ -142,7 +142,7 @@ int ipuma_compare_device(ipuma_engine *e, int32_t n, const uint8_t *codes, int64
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * Arguments and results as for ipuma_compare_device, forward pass only (the
  * engine must be forward-only, else IPUMA_EINVAL).  Its device work space is a
- * static array of about 205 MiB per GPU (held by every process that loads the
+ * static array of about 260 MiB per GPU (held by every process that loads the
  * library, until it exits), which bounds one call to n <= 65536 comparisons
  * (IPUMA_EINVAL beyond) and |A| <= 4096 (a longer A is reported as
  * IPUMA_ETOOLONG by ipuma_synchronize, and its score is written as -1).

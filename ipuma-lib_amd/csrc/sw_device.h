@@ -13,9 +13,19 @@ constexpr int kFastThreads = 768;
 // class c < 5 has L = 2^c, the last class L = 64.
 constexpr int kNumClasses = 6;
 constexpr int kClassL[kNumClasses] = {1, 2, 4, 8, 16, 64};
-// Boundary scratch of one wave, in 32-bit words: mpad * (64/L) * 2 <= this.
-// The class rules keep mpad * (64/L) <= 65536, so 512 KiB per wave.
-constexpr int64_t kWaveScratchWords = 65536 * 2;
+// Scratch of one wave, in 32-bit words: a group's boundary row (two words per
+// pipeline column, mpad * 64/L columns) and its column table (one word per
+// pipeline step and pipeline: at most (mpad + L + 4) * 64/L).
+__host__ __device__ constexpr int64_t wave_scratch_words(int64_t mpad, int lg) {
+  return (3 * mpad + (1 << lg) + 4) << (6 - lg);
+}
+// Boundary columns (mpad * 64/L) that a wave scratch of `words` words holds
+// for every pipeline depth: (3 * mpad + L + 4) * 64/L <= 3 * mpad * 64/L + 320.
+__host__ __device__ constexpr int64_t wave_cols_of(int64_t words) { return (words - 320) / 3; }
+// The class rules keep mpad * (64/L) <= 65536: 769 KiB per wave.
+constexpr int64_t kWaveScratchWords = 3 * 65536 + 320;
+// Boundary row of one band slot (a 64-lane group: two words per column).
+constexpr int64_t kBandScratchWords = 2 * 65536;
 // Longest sequence the engine accepts (positions are packed as uint16).
 constexpr int32_t kMaxSeqLen = 65535;
 // Counting-sort bins of the length bucketing: class x super-strip bin x
@@ -40,7 +50,7 @@ static_assert(kCostBins >= kCostBinsPerClass, "cost bins");
 // Band groups: a whole-wave pipeline group (L = 64) with at least two
 // super-strips is shared by the waves of one block, one super-strip per wave
 // at a time; a block keeps up to one band group per wave in flight, each with
-// its own boundary row of kWaveScratchWords words.
+// its own boundary row of kBandScratchWords words.
 constexpr int kBandSlots = kFastThreads / 64;
 constexpr int kMaxStrips = 64;  // super-strips of 1,024 rows cover 65,535 rows
 // Saturation value of the packed int16 path.
@@ -98,9 +108,10 @@ struct FastArgs {
   const GroupDesc *groups;     // in processing order
   BucketCtl *ctl;              // group ranges per class, error bits
   uint32_t *scratch;           // wave_words per wave
-  uint32_t *band_scratch;      // kBandSlots * wave_words per block
-  int64_t wave_words;          // boundary scratch of one wave or band slot, in words
-                               // (kWaveScratchWords, or less for a smaller scratch)
+  uint32_t *band_scratch;      // kBandSlots * band_words per block
+  int64_t wave_words;          // scratch of one wave, in words (kWaveScratchWords, or less
+                               // for a smaller scratch): boundary row + column table
+  int64_t band_words;          // boundary row of one band slot, in words
   int32_t *scores;
   int32_t *a_range;
   int32_t *b_range;
@@ -170,7 +181,7 @@ struct BucketArgs {
   int64_t target_slots;   // resident half-lanes x rounds: step cap = cells / (kRows * this)
   int32_t force_class;    // -1: choose per pair
   int32_t wave_cols;      // boundary columns a wave's scratch holds: a pair's pipeline is
-                          // deep enough that mpad * 64/L <= wave_cols (kWaveScratchWords / 2)
+                          // deep enough that mpad * 64/L <= wave_cols (wave_cols_of(words))
   int32_t max_a;          // longest A accepted (kMaxSeqLen, or less with a smaller scratch);
                           // a longer one is an error like a length past kMaxSeqLen
   // Start pass only: the forward scores.  A pair's super-strip bin then counts

@@ -194,7 +194,10 @@ struct ipuma_engine {
 namespace {
 constexpr int32_t kNaPairs = 65536;  // comparisons per call
 constexpr int32_t kNaMaxA = 4096;    // longest A: a boundary row of mpad * 64/L <= 4096 columns
-constexpr int64_t kNaWaveWords = 2 * (int64_t)kNaMaxA;  // per wave and per band slot (32 KiB)
+// per wave: that boundary row and the group's column table (49.25 KiB)
+constexpr int64_t kNaWaveWords = 3 * (int64_t)kNaMaxA + 320;
+static_assert(wave_cols_of(kNaWaveWords) == kNaMaxA, "wave scratch of the allocation-free entry");
+constexpr int64_t kNaBandWords = 2 * (int64_t)kNaMaxA;  // per band slot: its boundary row (32 KiB)
 constexpr int kNaBlocks = 256;         // persistent blocks of the packed kernel (one per CU)
 constexpr int kNaGenericBlocks = 256;  // waves of the int32 kernel
 constexpr int64_t kNaWorkStride = 2 * (int64_t)kNaMaxA;  // its boundary row per wave (32 KiB)
@@ -213,7 +216,7 @@ constexpr int64_t na_size(int part) {
     case kNaSorted: return ((int64_t)kNaPairs + kSortedPad) * 4;
     case kNaGtmp: case kNaGroupsPart: return kNaGroups * (int64_t)sizeof(GroupDesc);
     case kNaWave: return (int64_t)kNaBlocks * (kFastThreads / 64) * kNaWaveWords * 4;
-    case kNaBand: return (int64_t)kNaBlocks * kBandSlots * kNaWaveWords * 4;
+    case kNaBand: return (int64_t)kNaBlocks * kBandSlots * kNaBandWords * 4;
     case kNaWork: return (int64_t)kNaGenericBlocks * kNaWorkStride * 4;
     default: return 0;
   }
@@ -223,8 +226,8 @@ constexpr int64_t na_offset(int part) {
   for (int i = 0; i < part; ++i) o += na_align(na_size(i));
   return o;
 }
-constexpr int64_t kNaBytes = na_offset(kNaParts);  // about 205 MiB
-static_assert(kNaBytes < ((int64_t)256 << 20), "static scratch stays below 256 MiB");
+constexpr int64_t kNaBytes = na_offset(kNaParts);  // about 260 MiB
+static_assert(kNaBytes < ((int64_t)288 << 20), "static scratch stays below 288 MiB");
 
 // The array itself.  Host code never uses its name as an address: each GPU
 // holds its own copy, whose device address hipGetSymbolAddress gives per
@@ -347,10 +350,10 @@ struct ChainPtrs {
   int32_t *bin_of, *rank, *hist, *costhist, *sorted, *gbin, *grank, *ovf;
   BucketCtl *ctl;
   GroupDesc *gtmp, *groups;
-  uint32_t *wave, *band;  // boundary scratch: wave_words per wave / per band slot
+  uint32_t *wave, *band;  // wave_words per wave / band_words per band slot
   int32_t *work;          // int32 kernel: work_stride ints per wave
   int64_t work_stride;
-  int64_t wave_words;
+  int64_t wave_words, band_words;
   int32_t max_a;          // longest A accepted
   int fast_blocks, generic_blocks;
   int64_t target_slots;
@@ -385,7 +388,7 @@ int run_chain_on(ipuma_engine *e, const ChainPtrs &c, int32_t n, const uint8_t *
   BucketArgs B{};
   B.n = n; B.a_len = a_len; B.b_len = b_len; B.target_slots = c.target_slots;
   B.force_class = e->force_class;
-  B.wave_cols = (int32_t)(c.wave_words / 2); B.max_a = c.max_a;
+  B.wave_cols = (int32_t)wave_cols_of(c.wave_words); B.max_a = c.max_a;
   B.plain = e->batching == IPUMA_BATCH_PLAIN ? 1 : e->batching == IPUMA_BATCH_PLAIN_TILE ? 2 : 0;
   B.max_groups = n + kRuns; B.bin_of = c.bin_of; B.rank = c.rank;
   B.hist = c.hist; B.grank = c.grank;
@@ -397,6 +400,7 @@ int run_chain_on(ipuma_engine *e, const ChainPtrs &c, int32_t n, const uint8_t *
   A.codes = codes; A.a_off = a_off; A.a_len = a_len; A.b_off = b_off; A.b_len = b_len;
   A.table = e->d_table; A.sorted = c.sorted; A.groups = c.groups;
   A.ctl = ctl; A.scratch = c.wave; A.band_scratch = c.band; A.wave_words = c.wave_words;
+  A.band_words = c.band_words;
   A.scores = scores; A.a_range = a_range; A.b_range = b_range;
   A.ovf_list = c.ovf; A.ovf_count = &ctl->ovf_count;
   A.max_groups = n + kRuns;
@@ -432,7 +436,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
     const size_t scratch_bytes =
         (size_t)e->fast_blocks * (kFastThreads / 64) * kWaveScratchWords * 4;
     if ((rc = e->scratch.ensure(scratch_bytes))) return rc;
-    if ((rc = e->band_scratch.ensure((size_t)e->fast_blocks * kBandSlots * kWaveScratchWords * 4)))
+    if ((rc = e->band_scratch.ensure((size_t)e->fast_blocks * kBandSlots * kBandScratchWords * 4)))
       return rc;
   }
   ChainPtrs p{};
@@ -442,7 +446,7 @@ int run_chain(ipuma_engine *e, ChainBufs &c, int32_t n, const uint8_t *codes,
   p.ctl = c.ctl.as<BucketCtl>(); p.gtmp = c.gtmp.as<GroupDesc>(); p.groups = c.groups.as<GroupDesc>();
   p.wave = e->scratch.as<uint32_t>(); p.band = e->band_scratch.as<uint32_t>();
   p.work = e->work.as<int32_t>(); p.work_stride = kWorkStride;
-  p.wave_words = kWaveScratchWords; p.max_a = kMaxSeqLen;
+  p.wave_words = kWaveScratchWords; p.band_words = kBandScratchWords; p.max_a = kMaxSeqLen;
   p.fast_blocks = e->fast_blocks; p.generic_blocks = e->generic_blocks;
   p.target_slots = e->target_slots;
   return run_chain_on(e, p, n, codes, a_off, a_len, b_off, b_len, scores, a_range, b_range, st,
@@ -1112,6 +1116,7 @@ static int ipuma_compare_device_noalloc_body(ipuma_engine *e, int32_t n, const u
   p.work = reinterpret_cast<int32_t *>(at(kNaWork));
   p.work_stride = kNaWorkStride;
   p.wave_words = kNaWaveWords;
+  p.band_words = kNaBandWords;
   p.max_a = kNaMaxA;
   p.fast_blocks = std::min(e->fast_blocks, kNaBlocks);
   p.generic_blocks = kNaGenericBlocks;

@@ -136,7 +136,6 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
                                           const GroupDesc gd, uint32_t *__restrict__ wscr,
                                           int lane, int lg_rt, const SimdShare &share) {
   constexpr uint32_t PADC = KP - 1;
-  constexpr uint32_t COLSTRIDE = KP * KP * 4;  // bytes of one A-code row of the LDS table
   // pipeline depth L = 2^lg: fixed for the solo and whole-wave modes
   const int lg = MODE == kModeSolo ? 0 : MODE == kModeWave ? 6 : lg_rt;
   // Row-pair keys hold 15-bit columns: every column a lane computes (up to
@@ -180,6 +179,38 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   // ss * L * kRows + k * R .. + R).
   // weight of the group's last super-strip (its rows per lane)
   const int wlast = step_weight32(PAIRKEY ? last_strip_rows(gd.last_rows) : kRows);
+
+  // Column table of the group, behind the boundary row in the wave's scratch:
+  // entry (t, g) packs the LDS table offsets (in words) of the A codes at
+  // column t of pipeline g's two comparisons (the pad code past each |A|,
+  // codes outside the matrix clamped to it), for every step a lane 0 takes
+  // (t < round4(mpad + L - 1), plus one prefetched).  Built once per group by
+  // the pipeline's lanes, it serves every super-strip: lane 0 then loads one
+  // word per step instead of decoding, clamping and scaling two codes.
+  // One-lane pipelines decode their own codes instead: every lane is a lane 0
+  // there, and the table would add a load per lane and step to the boundary
+  // row's.
+  constexpr uint32_t KK = KP * KP;
+  const int ncols = ((mpad + L - 1 + 3) & ~3) + 1;
+  uint32_t *const ctab = wscr + (size_t)mpad * NG * 2;
+  if constexpr (MODE != kModeSolo) {
+    for (int w = k; 4 * w < ncols; w += L) {
+      const int t = 4 * w;
+      const uint32_t wl = t < m_lo ? a32_lo[w] : 0u;
+      const uint32_t wh = t < m_hi ? a32_hi[w] : 0u;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (t + u < ncols) {
+          const uint32_t cl = t + u < m_lo ? min((wl >> (8 * u)) & 0xffu, PADC) : PADC;
+          const uint32_t ch = t + u < m_hi ? min((wh >> (8 * u)) & 0xffu, PADC) : PADC;
+          ctab[(t + u) * NG + g] = cl * KK | (ch * KK) << 16;
+        }
+      }
+    }
+    // the entries are read by lane 0 of each pipeline: complete the stores
+    // and drop stale L1 lines
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+  }
   auto strip = [&](auto rows_per_lane, const int ss) {
     constexpr int R = decltype(rows_per_lane)::value;
     constexpr int KN = PAIRKEY ? R / 2 : R;
@@ -219,16 +250,19 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
       KH[q] = 0u;
     }
     uint32_t out_h = 0u, out_f = 0u, hdiag = 0u;
-    uint32_t col_lo = PADC * COLSTRIDE, col_hi = PADC * COLSTRIDE;
+    uint32_t colw = PADC * KK * 0x10001u;  // the column's table offsets: low | high << 16
     const bool load_bnd = ss > 0;
     const bool store_bnd = ss + 1 < nss;
 
     uint2 nb = make_uint2(0u, 0u);
     if (first && load_bnd && mpad > 0) nb = *(const uint2 *)(wscr + (size_t)g * 2);
-    uint32_t na_lo = 0u, na_hi = 0u;
-    if (first) {
+    uint32_t ncw = 0u;
+    uint32_t na_lo = 0u, na_hi = 0u;  // one-lane pipelines: the next four A codes
+    if constexpr (MODE == kModeSolo) {
       na_lo = (0 < m_lo) ? a32_lo[0] : 0u;
       na_hi = (0 < m_hi) ? a32_hi[0] : 0u;
+    } else {
+      if (first) ncw = ctab[g];
     }
     const int T = mpad + L - 1;
     for (int t0 = 0; t0 < T; t0 += 4) {
@@ -236,7 +270,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         share_simd(share, ((nss - 1 - ss) * 32 - 32 + wlast) * T + (T - t0) * step_weight32(R) +
                               (ss + 1 == nss ? (32 - wlast) * T : 0), lane);
       const uint32_t wa_lo = na_lo, wa_hi = na_hi;
-      if (first) {
+      if constexpr (MODE == kModeSolo) {
         const int tn = t0 + 4;
         na_lo = (tn < m_lo) ? a32_lo[tn >> 2] : 0u;
         na_hi = (tn < m_hi) ? a32_hi[tn >> 2] : 0u;
@@ -245,17 +279,25 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
       for (int u = 0; u < 4; ++u) {
         const int t = t0 + u;
         const uint2 cb = nb;
-        // byte offsets into the wave scratch fit 32 bits (it holds 512 KiB)
+        const uint32_t ccw = ncw;
+        // t + 1 < ncols: the table covers every step of the loop plus one
+        if (MODE != kModeSolo && first) ncw = ctab[(t + 1) * NG + g];
+        // byte offsets into the wave scratch fit 32 bits (it holds 769 KiB)
         if (first && load_bnd && t + 1 < mpad)
           nb = *(const uint2 *)((const char *)wscr + (uint32_t)(((t + 1) * NG + g) * 8));
         else
           nb = make_uint2(0u, 0u);
-        const uint32_t ac_lo = (t < m_lo) ? min((wa_lo >> (8 * u)) & 0xffu, PADC) : PADC;
-        const uint32_t ac_hi = (t < m_hi) ? min((wa_hi >> (8 * u)) & 0xffu, PADC) : PADC;
         const uint32_t rh = from_prev<MODE>(cb.x, out_h, first);
         const uint32_t rf = from_prev<MODE>(cb.y, out_f, first);
-        col_lo = from_prev<MODE>(ac_lo * COLSTRIDE, col_lo, first);
-        col_hi = from_prev<MODE>(ac_hi * COLSTRIDE, col_hi, first);
+        uint32_t col_lo, col_hi;  // byte offsets of the column's A-code rows of the table
+        if constexpr (MODE == kModeSolo) {
+          col_lo = ((t < m_lo) ? min((wa_lo >> (8 * u)) & 0xffu, PADC) : PADC) * (KK * 4);
+          col_hi = ((t < m_hi) ? min((wa_hi >> (8 * u)) & 0xffu, PADC) : PADC) * (KK * 4);
+        } else {
+          colw = from_prev<MODE>(ccw, colw, first);
+          col_lo = (colw & 0xFFFFu) * 4u;
+          col_hi = (colw >> 16) * 4u;
+        }
         const int j = t - k;
         // key low parts: 16-bit 0xFFFF - j, or with row pairs the preferred
         // (even) row's bit 15 over a 15-bit 0x7FFF - j
@@ -497,7 +539,7 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
   const int nss = __builtin_amdgcn_readfirstlane(S.gd.nss);
   const int gbase = __builtin_amdgcn_readfirstlane(S.gd.base);
   uint32_t *__restrict__ bscr =
-      A.band_scratch + ((int64_t)blockIdx.x * kBandSlots + sl) * A.wave_words;
+      A.band_scratch + ((int64_t)blockIdx.x * kBandSlots + sl) * A.band_words;
 
   const int p_lo = A.sorted[gbase];
   const int p_hi = A.sorted[gbase + 1];
@@ -756,7 +798,7 @@ __device__ void band_load(const FastArgs &A, BandState &B, int cur, int lane, in
     if (tk >= nband) break;
     const GroupDesc gd = A.groups[tk];
     const bool ok = gd.cls == kNumClasses - 1 && gd.base >= 0 && gd.mpad > 0 &&
-                    gd.mpad <= kMaxSeqLen && (int64_t)gd.mpad * 2 <= A.wave_words &&
+                    gd.mpad <= kMaxSeqLen && (int64_t)gd.mpad * 2 <= A.band_words &&
                     gd.nss >= 2 && gd.nss <= kMaxStrips;
     if (!ok) {
       atomicOr(&A.ctl->error, 4);
@@ -879,12 +921,13 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
   for (int gi = next_group(&A.ctl->queue, nband, ng, lane); gi >= 0;
        gi = next_group(&A.ctl->queue, nband, ng, lane)) {
     const GroupDesc gd = A.groups[gi];
-    // a descriptor outside what the bucketing produces (class, scratch row
-    // mpad * 64/L * 2 <= wave_words, rows <= kMaxSeqLen) is reported and skipped
+    // a descriptor outside what the bucketing produces (class, boundary row
+    // and column table within the wave's scratch, rows <= kMaxSeqLen) is
+    // reported and skipped
     const int cls = gd.cls;
     const int lg = cls < kNumClasses - 1 ? cls : 6;
     const bool ok = cls >= 0 && cls < kNumClasses && gd.base >= 0 && gd.mpad >= 0 &&
-                    gd.mpad <= kMaxSeqLen && ((int64_t)gd.mpad << (7 - lg)) <= A.wave_words &&
+                    gd.mpad <= kMaxSeqLen && wave_scratch_words(gd.mpad, lg) <= A.wave_words &&
                     gd.nss >= 0 && ((int64_t)(gd.nss - 1) << lg) * kRows < kMaxSeqLen &&
                     gd.last_rows >= 1 && gd.last_rows <= kRows;
     if (!ok) {

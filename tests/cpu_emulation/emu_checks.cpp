@@ -66,17 +66,18 @@ int main() {
     A.b_range = &br; A.ovf_list = ovf_list; A.ovf_count = &ovf;
     A.max_groups = 2; A.gap_o = 0x00010001u; A.gap_x = 0x00010001u;
     A.wave_words = kWaveScratchWords;
+    A.band_words = kBandScratchWords;
     launch_fast(kp, A, 1, nullptr);
     CHECK(ctl.error & 4, "corrupted descriptor raises error bit 4");
     CHECK(score == 8, "the valid group still runs (8 matches)");
-    // 3. a group whose boundary row would not fit a smaller wave scratch:
-    // one-lane pipelines of mpad 16 need 16 * 64 * 2 = 2048 words, the real
-    // group (mpad 8) 1024
+    // 3. a group whose boundary row and column table would not fit a smaller
+    // wave scratch: one-lane pipelines of mpad 16 need (3 * 16 + 5) * 64 =
+    // 3392 words, the real group (mpad 8) (3 * 8 + 5) * 64 = 1856
     std::memset(&ctl, 0, sizeof(ctl));
     ctl.ngroups = 2;
     gd[0] = {0, 16, 2, 0, 16};
     score = -5;
-    A.wave_words = 1024;
+    A.wave_words = 2048;
     launch_fast(kp, A, 1, nullptr);
     CHECK(ctl.error & 4, "a boundary row past the wave scratch raises error bit 4");
     CHECK(score == 8, "the group that fits still runs");
