@@ -126,6 +126,7 @@ struct FastArgs {
   const int32_t *target;
   uint32_t gap_o;              // -(gapInit + gapExtend), packed twice
   uint32_t gap_x;              // -gapExtend, packed twice
+  int32_t smax;                // largest substitution score, at least 0
 };
 
 // Arguments of the exact int32 kernel.

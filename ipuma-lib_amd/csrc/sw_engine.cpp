@@ -124,6 +124,7 @@ struct ipuma_engine {
   double clock_hz = 0.0;  // peak shader clock
   int kp = 0;   // padded alphabet (matrix side + pad code)
   int dim = 0;  // matrix side
+  int smax = 0; // largest entry of the score table, at least 0
   hipStream_t stream = nullptr;
   std::vector<int16_t> h_table;
   uint8_t h_codes[256];
@@ -297,6 +298,7 @@ int init_engine(ipuma_engine *e) {
   e->kp = dim + 1;
   e->h_table.resize((size_t)e->kp * e->kp);
   build_pad_table(mat.data(), dim, e->h_table.data());
+  e->smax = std::max<int>(0, *std::max_element(e->h_table.begin(), e->h_table.end()));
   build_code_table(e->sw.datatype, e->h_codes);
   // codes outside the matrix (e.g. a nucleic-acid encoding with an amino-acid
   // matrix) would index past the table: clamp them to the last real code.
@@ -409,6 +411,7 @@ int run_chain_on(ipuma_engine *e, const ChainPtrs &c, int32_t n, const uint8_t *
   const uint32_t x = (uint32_t)(-e->sw.gap_extend) & 0xFFFFu;
   A.gap_o = o | (o << 16);
   A.gap_x = x | (x << 16);
+  A.smax = e->smax;
   if (time_fast) HIP_TRY(hipEventRecord(e->ev[1], st));
   HIP_TRY(launch_fast(e->kp, A, c.fast_blocks, st));
   if (time_fast) HIP_TRY(hipEventRecord(e->ev[2], st));

@@ -47,6 +47,8 @@ constexpr int kModeSolo = 0;
 // run to mpad + 62 and must fit 15 bits.
 constexpr int kPairKeyMaxM = 32768 - 64;
 constexpr int kModeRow = 1;
+// Largest H a row-tag key holds: H * 16 + 15 fits 16 bits.
+constexpr int kTagKeyMaxH = 4095;
 constexpr int kModeWave = 2;
 
 // Value of lane k-1 of the same pipeline (k = lane % L); the first lane of a
@@ -102,11 +104,18 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
 // random banks, which costs less than the address arithmetic a per-row
 // lookup needs.
 //
-// Position tracking: per row, key = H << 16 | (0xFFFF - j); the max key is the
-// row maximum at its first column.  Pipelines of up to 16 lanes share one key
-// between rows 2q and 2q+1: H << 16 | (row 2q ? 1 : 0) << 15 | (0x7FFF - j),
-// which orders (max H, min row, min column) inside the pair.  Rows merge in
-// order with a strict >, lanes merge at the end by (max H, min row).
+// Position tracking, row-tag keys: while every H of a super-strip stays below
+// 4096 (the best H above it plus the largest score times its rows, checked
+// for the whole wave before the super-strip), each row's packed H becomes a
+// 16-bit key H * 16 + (15 - r) (one v_pk_mad_u16 for both comparisons), a
+// packed max over the lane's rows gives (max H, first row) of the step, and
+// one 32-bit key per comparison, that << 16 | (0xFFFF - j), merges the steps:
+// the max key is the lane's first maximum in B-major order.  Otherwise, per
+// row, key = H << 16 | (0xFFFF - j); pipelines of up to 16 lanes share one
+// key between rows 2q and 2q+1: H << 16 | (row 2q ? 1 : 0) << 15 |
+// (0x7FFF - j), which orders (max H, min row, min column) inside the pair, and
+// rows merge in order with a strict >.  Lanes merge at the end by (max H, min
+// row).
 // ---------------------------------------------------------------------------
 // Issue priority among the waves of one SIMD.  By default the oldest wave
 // wins every issue conflict, so the youngest wave on a SIMD barely advances
@@ -211,9 +220,15 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     // and drop stale L1 lines
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
   }
-  auto strip = [&](auto rows_per_lane, const int ss) {
+  // TAG: row-tag keys (the super-strip's H values stay below kTagKeyMaxH + 1):
+  // per step, a packed 16-bit key H * 16 + (15 - r) per row, a max over the
+  // rows (both comparisons at once), then one 32-bit key per comparison.
+  auto strip = [&](auto rows_per_lane, auto tag_keys, const int ss) {
     constexpr int R = decltype(rows_per_lane)::value;
-    constexpr int KN = PAIRKEY ? R / 2 : R;
+    constexpr bool TAG = decltype(tag_keys)::value;
+    constexpr bool PK = PAIRKEY && !TAG;
+    static_assert(!TAG || (PAIRKEY && R <= 16), "row tags: 4 bits, 16-bit columns");
+    constexpr int KN = TAG ? 1 : PAIRKEY ? R / 2 : R;
     // The boundary row written by lane L-1 during the previous super-strip is
     // read back by lane 0: complete those stores and drop stale L1 lines.
     if (ss > 0) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
@@ -301,8 +316,8 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         const int j = t - k;
         // key low parts: 16-bit 0xFFFF - j, or with row pairs the preferred
         // (even) row's bit 15 over a 15-bit 0x7FFF - j
-        const uint32_t jinv = PAIRKEY ? ((0x7FFFu - (uint32_t)j) & 0x7FFFu) | 0x8000u
-                                      : 0xFFFFu - (uint32_t)j;
+        const uint32_t jinv = PK ? ((0x7FFFu - (uint32_t)j) & 0x7FFFu) | 0x8000u
+                                 : 0xFFFFu - (uint32_t)j;
         const char *tl = (const char *)tab + col_lo;
         const char *th = (const char *)tab + col_hi;
         // keep the row-pair offsets packed two per register across the loop
@@ -328,7 +343,10 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         // VGPRs (otherwise they are re-derived per row, or a uniform tail
         // lands in an SGPR where v_and_or_b32 cannot take it with its mask)
         uint32_t jr_even = jinv, jr_odd = jinv & 0x7FFFu;
-        if constexpr (PAIRKEY) asm volatile("" : "+v"(jr_even), "+v"(jr_odd));
+        if constexpr (PK) asm volatile("" : "+v"(jr_even), "+v"(jr_odd));
+        u16x2 tk2[2] = {(u16x2)0, (u16x2)0};  // row-tag keys: running maxima over the rows
+        uint32_t tag_mul = 0x00100010u;  // 16 in both halves
+        if constexpr (TAG) asm volatile("" : "+v"(tag_mul));
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           s16x2 t2n = t2;
@@ -339,7 +357,16 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
           const u16x2 ho = __builtin_elementwise_sub_sat(U2(hn), U2(gap_o));
           E[r] = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))));
           f = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(f), U2(gap_x))));
-          if constexpr (PAIRKEY) {
+          if constexpr (TAG) {
+            // H < 4096 in both halves: H * 16 + (15 - r) in 16 bits orders
+            // (max H, min row) within the step.  One v_pk_mad_u16 per row:
+            // the multiplier is opaque to the compiler (a shift and an or
+            // would take two instructions, and the tag constants registers)
+            const u16x2 kr = U2(hn) * U2(tag_mul) + (unsigned short)(15 - r);
+            // two running maxima (even and odd rows) keep the chain short
+            if (r < 2) tk2[r] = kr;
+            else tk2[r & 1] = __builtin_elementwise_max(tk2[r & 1], kr);
+          } else if constexpr (PAIRKEY) {
             // H << 16 | row bit << 15 | 15-bit column; one max3 per row pair
             const uint32_t jr = (r & 1) ? jr_odd : jr_even;
             const uint32_t kl = (hn << 16) | jr;
@@ -357,6 +384,12 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
           }
           t2 = t2n;
         }
+        if constexpr (TAG) {
+          // step key over 16-bit columns: (max H, min row) << 16 | 0xFFFF - j
+          const u16x2 tk = __builtin_elementwise_max(tk2[0], tk2[1]);
+          KL[0] = max(KL[0], __builtin_amdgcn_perm(W(tk), jinv, 0x05040100u));
+          KH[0] = max(KH[0], __builtin_amdgcn_perm(W(tk), jinv, 0x07060100u));
+        }
         out_h = H[R - 1];
         out_f = f;
         if (last && store_bnd && j >= 0 && j < mpad)
@@ -364,6 +397,21 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
       }
     }
     // rows (or row pairs) in order, strict >: the first maximum in B order
+    if constexpr (TAG) {
+      const int hl = (int)(KL[0] >> 20);
+      if (hl > bh_lo) {
+        bh_lo = hl;
+        bi_lo = row0 + 15 - (int)((KL[0] >> 16) & 15u);
+        bj_lo = 0xFFFF - (int)(KL[0] & 0xFFFFu);
+      }
+      const int hh = (int)(KH[0] >> 20);
+      if (hh > bh_hi) {
+        bh_hi = hh;
+        bi_hi = row0 + 15 - (int)((KH[0] >> 16) & 15u);
+        bj_hi = 0xFFFF - (int)(KH[0] & 0xFFFFu);
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < KN; ++q) {
       const int hl = (int)(KL[q] >> 16);
@@ -396,9 +444,26 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   // push the forward kernel's step loops into spilling)
   const int32_t *const target = A.target;
   using Full = std::integral_constant<int, kRows>;
+  // Key form of a super-strip, uniform over the wave.  A path into the
+  // super-strip gains at most the largest score per row it adds (gaps cost),
+  // so every H in it is at most the best H above it (a lane's bh bounds its
+  // pipeline's best only together with the other lanes: the ballot covers
+  // every lane) plus A.smax times its rows.  Row-tag keys hold when that stays
+  // within kTagKeyMaxH for every pair of the wave.
+  auto run_strip = [&](auto rows, const int ss) {
+    constexpr int R = decltype(rows)::value;
+    if constexpr (PAIRKEY) {
+      const int lim = kTagKeyMaxH - A.smax * (R << lg);
+      if (__ballot(max(bh_lo, bh_hi) > lim) == 0) {
+        strip(rows, std::true_type{}, ss);
+        return;
+      }
+    }
+    strip(rows, std::false_type{}, ss);
+  };
   bool reached = false;
   for (int ss = 0; ss + 1 < nss; ++ss) {
-    strip(Full{}, ss);
+    run_strip(Full{}, ss);
     if constexpr (STOP) {
       const int tg_lo = p_lo >= 0 ? target[p_lo] : INT32_MIN;
       const int tg_hi = p_hi >= 0 ? target[p_hi] : INT32_MIN;
@@ -414,11 +479,11 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   if (nss > 0 && !reached) {
     if constexpr (PAIRKEY) {
       const int rl = last_strip_rows(gd.last_rows);
-      if (rl == 8) strip(std::integral_constant<int, 8>{}, nss - 1);
-      else if (rl == 12) strip(std::integral_constant<int, 12>{}, nss - 1);
-      else strip(Full{}, nss - 1);
+      if (rl == 8) run_strip(std::integral_constant<int, 8>{}, nss - 1);
+      else if (rl == 12) run_strip(std::integral_constant<int, 12>{}, nss - 1);
+      else run_strip(Full{}, nss - 1);
     } else {
-      strip(Full{}, nss - 1);
+      run_strip(Full{}, nss - 1);
     }
   }
 
@@ -526,7 +591,7 @@ __device__ __forceinline__ int band_wait(int *prog, int need) {
 // Super-strip s of the band group in slot `sl`: the whole-wave pipeline body
 // of run_group, with the boundary row shared through the slot's buffer.
 template <int KP, bool PAIRKEY>
-__device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
+__device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
                            int sl, int s, int lane) {
   constexpr int R = kRows;
   constexpr uint32_t PADC = KP - 1;
@@ -791,7 +856,7 @@ __device__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, 
 
 // Loads the next band group into a free slot and makes it current (called by
 // the wave that exhausted group `cur`), or marks the band phase finished.
-__device__ void band_load(const FastArgs &A, BandState &B, int cur, int lane, int nband) {
+__device__ __forceinline__ void band_load(const FastArgs &A, BandState &B, int cur, int lane, int nband) {
   const int seq = ((cur >> 4) + 1) & 0xFFFF;
   for (;;) {
     const int tk = __builtin_amdgcn_readfirstlane(atomicAdd(&A.ctl->band_queue, lane == 0 ? 1 : 0));
@@ -830,7 +895,7 @@ __device__ void band_load(const FastArgs &A, BandState &B, int cur, int lane, in
 // The band phase of a block: its waves take strips of the current band group
 // until no band group is left.
 template <int KP>
-__device__ void band_phase(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
+__device__ __forceinline__ void band_phase(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
                            int lane, int nband) {
   for (;;) {
     const int cur = lds_load_acq(&B.cur);

@@ -294,6 +294,63 @@ def test_band_groups_emulated(emu_lib):
         eng.close()
 
 
+def _tie_pair(rng, n):
+    """A = X + Y, B = Y + X with |X| = |Y| = n: two equal best alignments, the
+    one on the earlier B row ending at the later A column."""
+    x = "".join(rng.choice(list("ACGT"), n))
+    y = "".join(rng.choice(list("ACGT"), n))
+    return x + y, y + x
+
+
+def test_row_tag_keys_emulated(emu_lib):
+    """Super-strips whose H values stay within 12 bits take 16-bit row-tag keys
+    (H * 16 + 15 - row, a max over the step's rows); the rest the 32-bit keys.
+    A super-strip of R rows can raise H by at most R times the largest score,
+    so a nearly identical pair scoring above 4,095 crosses from one form to the
+    other mid-pair: match 7 at depths 1-16, match 3 at depth 64 (whose first
+    super-strip, 1,024 rows, already passes 4,095 - 3 * 1,024).  Tie pairs put
+    two equal maxima in one lane's rows with the earlier row at the later
+    column; short random pairs tie everywhere.  Scores and both ranges, against
+    the oracle."""
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(77)
+
+    def batch(m):
+        a = "".join(rng.choice(list("ACGT"), m))
+        mut = "".join(c if rng.random() > 0.01 else str(rng.choice(list("ACGT"))) for c in a)
+        A, B = [a], ["".join(rng.choice(list("ACGT"), 7)) + mut]
+        for n in (3, 5, 7, 8, 12, 30):
+            a, b = _tie_pair(rng, n)
+            A.append(a)
+            B.append(b)
+        for i in range(12):
+            A.append("".join(rng.choice(list("ACGT"), int(rng.integers(1, 40)))))
+            B.append("".join(rng.choice(list("ACGT"), int(rng.integers(1, 90)))))
+        return A, B
+
+    cases = ((dict(gapInit=-6, gapExtend=-2, matchValue=7, mismatchValue=-4), batch(640),
+              (1, 4, 16)),
+             (dict(gapInit=-4, gapExtend=-1, matchValue=3, mismatchValue=-2), batch(1100), (64,)))
+    for cfg, (A, B), depths in cases:
+        exp = O.sw_batch(A, B, gap_init=cfg["gapInit"], gap_extend=cfg["gapExtend"],
+                         match=cfg["matchValue"], mismatch=cfg["mismatchValue"],
+                         forward_only=False)
+        # the long pair passes the row-tag bound of its deepest super-strips
+        assert int(np.max(exp[0])) > 4095 - cfg["matchValue"] * 16 * max(depths)
+        for lanes in depths:
+            eng = I.SWAlgorithm(I.SWConfig(**cfg),
+                                I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+            eng.set_lanes_per_pair(lanes)
+            got = eng.compare(A, B)
+            for g, e, what in zip(got, exp, ("scores", "a_range", "b_range")):
+                g = np.asarray(g)
+                bad = np.nonzero(g != np.asarray(e))[0]
+                assert bad.size == 0, (cfg, lanes, what, bad[:5], g[bad[:5]],
+                                       np.asarray(e)[bad[:5]])
+            eng.close()
+
+
 def test_prepared_buffers_emulated(emu_lib):
     """The wire-format path through the emulated engine: prepare_remote (fill
     first and greedy), prepared_remote_compare, transferResults, against the
