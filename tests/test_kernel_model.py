@@ -7,7 +7,11 @@ boundary row, E and F clamped at 0 with unsigned saturating subtraction, the
 substitution add saturating at 32767, pad code scores of -32768, per-row keys
 H << 16 | (0xFFFF - j) -- or, for pipelines of up to 16 lanes, one key per
 row pair, H << 16 | (even row) << 15 | (0x7FFF - j) -- rows merged with a
-strict > and lanes by (max H, min row).  Agreement with the oracle on random inputs -- including pipeline fill
+strict > and lanes by (max H, min row).  With tagkey, a super-strip whose H
+values are bounded below 4096 (the best H of the lanes so far plus the
+largest score times the super-strip's rows) takes row-tag keys instead: per
+step and lane the max over rows of H * 16 + (15 - r), merged over the steps as
+that << 16 | (0xFFFF - j); the model asserts the bound on every cell.  Agreement with the oracle on random inputs -- including pipeline fill
 and drain, padded rows and columns, and the saturation flag -- checks the
 algebra of the kernel before it ever runs on a device.
 """
@@ -19,7 +23,7 @@ import oracle as O
 SAT = 32767
 
 
-def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False):
+def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
     m, n = len(a), len(b)
     o, x = -(gI + gE), -gE
     dim = mat.shape[0]
@@ -31,6 +35,8 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False):
     bnd_h = np.zeros(mpad, np.int64)
     bnd_f = np.zeros(mpad, np.int64)
     best = [(0, 0, 0)] * L  # per lane (h, i, j)
+    smax = max(0, int(np.max(mat)))
+    tags_used = 0
 
     def subsat(v, c):
         return max(v - c, 0)
@@ -42,6 +48,9 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False):
         E = np.zeros((L, R), np.int64)
         key = np.zeros((L, R), np.int64)
         pkey = np.zeros((L, (R + 1) // 2), np.int64)
+        use_tag = tagkey and R <= 16 and max(bh for bh, _, _ in best) + smax * R * L <= 4095
+        tags_used += use_tag
+        tkey = [0] * L
         out_h = [0] * L
         out_f = [0] * L
         col = [pad] * L
@@ -66,6 +75,7 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False):
                 d = hdiag[k]
                 hdiag[k] = rh
                 f = rf
+                k16 = 0
                 for r in range(R):
                     s = table[col[k], bcode[k][r]]
                     t2 = min(d + s, SAT)
@@ -76,8 +86,13 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False):
                     key[k, r] = max(key[k, r], (hn << 16) | jinv)
                     pkey[k, r // 2] = max(pkey[k, r // 2],
                                           (hn << 16) | ((r % 2 == 0) << 15) | j15)
+                    if use_tag:
+                        assert hn <= 4095, (hn, ss)  # the super-strip's bound
+                        k16 = max(k16, hn * 16 + 15 - r)
                     d = H[k, r]
                     H[k, r] = hn
+                if use_tag:
+                    tkey[k] = max(tkey[k], (k16 << 16) | jinv)
                 out_h[k] = int(H[k, R - 1])
                 out_f[k] = int(f)
                 if k == L - 1 and 0 <= j < mpad:
@@ -85,6 +100,12 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False):
                     new_bf[j] = out_f[k]
         bnd_h, bnd_f = new_bh, new_bf
         for k in range(L):
+            if use_tag:
+                h = tkey[k] >> 20
+                if h > best[k][0]:
+                    best[k] = (h, rows[k][15 - ((tkey[k] >> 16) & 15)],
+                               0xFFFF - (tkey[k] & 0xFFFF))
+                continue
             if pairkey:
                 for q in range((R + 1) // 2):
                     h = int(pkey[k, q] >> 16)
@@ -100,6 +121,7 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False):
     for bh, bi, bj in best:
         if bh > h or (bh == h and bi < i):
             h, i, j = bh, bi, bj
+    model.tags_used = tags_used
     return h, i, j
 
 
@@ -148,3 +170,35 @@ def test_model_flags_saturation():
     a = np.zeros(300, np.int64)  # 300 * 127 = 38100 > 32767
     h, _, _ = model(a, a, mat, -10, -1, L=2, R=8)
     assert h == SAT
+
+
+@pytest.mark.parametrize("L,R", [(1, 16), (2, 8), (4, 4), (3, 12)])
+def test_model_row_tag_keys(L, R):
+    """Row-tag keys where the super-strip bound allows them, 32-bit row-pair
+    keys elsewhere: short pairs (ties everywhere), and pairs scoring above
+    4,095 whose later super-strips switch forms (match 7); the model asserts
+    that no H of a row-tag super-strip passes 4,095."""
+    rng = np.random.default_rng(100 + L * 16 + R)
+    for trial in range(14):
+        if trial < 2:
+            match, m = 7, int(rng.integers(600, 700))
+            a = rng.choice([1, 2, 3, 4], m).astype(np.int64)
+            b = np.concatenate([rng.choice([1, 2, 3, 4], 5), a]).astype(np.int64)
+        else:
+            match = int(rng.integers(1, 4))
+            m, n = rng.integers(1, 40, 2)
+            a = rng.choice([1, 2, 3, 4], m).astype(np.int64)
+            b = rng.choice([1, 2, 3, 4], n).astype(np.int64)
+            if trial % 3 == 0:  # A = X + Y, B = Y + X: equal maxima, earlier row at the later column
+                x, y = a[: m // 2], a[m // 2:]
+                a, b = np.concatenate([x, y]), np.concatenate([y, x])
+        mat = O.select_matrix(O.SIM_NA, match, -1, -1)
+        gI, gE = -2, -1
+        h, i, j = model(a, b, mat, gI, gE, L, R, len(a) + int(rng.integers(0, 5)), pairkey=True,
+                        tagkey=True)
+        if trial < 2:
+            assert model.tags_used >= 1 and h > 4095 - match * R * L
+        codes = np.concatenate([a, b]).astype(np.uint8)
+        s, ar, br = O.sw_batch_codes(codes, [0], [len(a)], [len(a)], [len(b)], mat, gI, gE, True)
+        assert (h, j, i) == (int(s[0]), O.unpack_range(ar[0])[1] if h else 0,
+                             O.unpack_range(br[0])[1] if h else 0), (trial, match)
