@@ -235,7 +235,8 @@ hipError_t launch_reverse_prefix(int32_t n, const uint8_t *codes, const int64_t 
 hipError_t launch_search_pairs(int32_t nq, int32_t ndb, const int64_t *q_off,
                                const int32_t *q_len, const int64_t *db_off,
                                const int32_t *db_len, int64_t *a_off, int32_t *a_len,
-                               int64_t *b_off, int32_t *b_len, hipStream_t st);
+                               int64_t *b_off, int32_t *b_len, int32_t query_cols,
+                               hipStream_t st);
 hipError_t launch_topk(int32_t nq, int32_t ndb, int32_t k, const int32_t *scores,
                        int32_t *top_idx, int32_t *top_score, hipStream_t st);
 hipError_t launch_finish_starts(int32_t n, const int32_t *scores, const int32_t *ra_range,

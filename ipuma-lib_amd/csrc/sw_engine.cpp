@@ -125,6 +125,7 @@ struct ipuma_engine {
   int kp = 0;   // padded alphabet (matrix side + pad code)
   int dim = 0;  // matrix side
   int smax = 0; // largest entry of the score table, at least 0
+  bool symmetric = false;  // the score table equals its transpose
   hipStream_t stream = nullptr;
   std::vector<int16_t> h_table;
   uint8_t h_codes[256];
@@ -299,6 +300,10 @@ int init_engine(ipuma_engine *e) {
   e->h_table.resize((size_t)e->kp * e->kp);
   build_pad_table(mat.data(), dim, e->h_table.data());
   e->smax = std::max<int>(0, *std::max_element(e->h_table.begin(), e->h_table.end()));
+  e->symmetric = true;
+  for (int i = 0; i < e->kp; ++i)
+    for (int j = 0; j < i; ++j)
+      if (e->h_table[(size_t)i * e->kp + j] != e->h_table[(size_t)j * e->kp + i]) e->symmetric = false;
   build_code_table(e->sw.datatype, e->h_codes);
   // codes outside the matrix (e.g. a nucleic-acid encoding with an amino-acid
   // matrix) would index past the table: clamp them to the last real code.
@@ -1165,8 +1170,10 @@ static int ipuma_search_device_body(ipuma_engine *e, const uint8_t *codes, int64
   int32_t *res = e->s_res.as<int32_t>();
   int32_t *sc = scores ? scores : res + 2 * n;
   if ((rc = begin_work(e, st))) return rc;
+  // a symmetric table scores (query, sequence) as (sequence, query): the query
+  // then runs over the columns, shared by both comparisons of a lane
   HIP_TRY(launch_search_pairs(nq, ndb, q_off, q_len, db_off, db_len, a_off, a_len, b_off, b_len,
-                              st));
+                              e->symmetric ? 1 : 0, st));
   // scores only: the forward pass (its end positions are discarded)
   const int saved = e->forward_only;
   e->forward_only = 1;

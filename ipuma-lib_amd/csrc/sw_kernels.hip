@@ -102,7 +102,9 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
 // permutes give both halves' scores of a row pair.  The table (KP^3 words,
 // 70 KB for the 26-letter protein alphabet) is not replicated: its loads hit
 // random banks, which costs less than the address arithmetic a per-row
-// lookup needs.
+// lookup needs.  Where both comparisons of every pipeline read the same A (a
+// search's query), word (a, b_lo, b_hi) is a row's pair of scores itself: one
+// load per row, no permutes.
 //
 // Position tracking, row-tag keys: while every H of a super-strip stays below
 // 4096 (the best H above it plus the largest score times its rows, checked
@@ -180,6 +182,16 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   const uint32_t *a32_lo = (const uint32_t *)a_lo;
   const uint32_t *a32_hi = (const uint32_t *)a_hi;
   const uint32_t gap_o = A.gap_o, gap_x = A.gap_x;
+  // Shared A: both comparisons of every pipeline align the same A (a search's
+  // query), so the row-pair table word (a, b_lo, b_hi) holds a row's scores
+  // for both halves and one load replaces two loads and two permutes.  A
+  // pipeline with no comparison, or only a low one, shares trivially.
+  bool shared = false;
+  if constexpr (MODE == kModeRow) {
+    const bool same =
+        (p_lo < 0 && p_hi < 0) || (p_lo >= 0 && (p_hi < 0 || (a_lo == a_hi && m_lo == m_hi)));
+    shared = __ballot(!same) == 0;
+  }
 
   int bh_lo = 0, bi_lo = 0, bj_lo = 0, bh_hi = 0, bi_hi = 0, bj_hi = 0;
 
@@ -223,9 +235,10 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   // TAG: row-tag keys (the super-strip's H values stay below kTagKeyMaxH + 1):
   // per step, a packed 16-bit key H * 16 + (15 - r) per row, a max over the
   // rows (both comparisons at once), then one 32-bit key per comparison.
-  auto strip = [&](auto rows_per_lane, auto tag_keys, const int ss) {
+  auto strip = [&](auto rows_per_lane, auto tag_keys, auto shared_a, const int ss) {
     constexpr int R = decltype(rows_per_lane)::value;
     constexpr bool TAG = decltype(tag_keys)::value;
+    constexpr bool SH = decltype(shared_a)::value;
     constexpr bool PK = PAIRKEY && !TAG;
     static_assert(!TAG || (PAIRKEY && R <= 16), "row tags: 4 bits, 16-bit columns");
     constexpr int KN = TAG ? 1 : PAIRKEY ? R / 2 : R;
@@ -250,8 +263,14 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         ch[r] = (row0 + r < n_hi) ? min((wh[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
       }
 #pragma unroll
-      for (int q = 0; q < R / 2; ++q)
-        rp[q] = ((cl[2 * q] * KP + cl[2 * q + 1]) * 4u) | (((ch[2 * q] * KP + ch[2 * q + 1]) * 4u) << 16);
+      for (int q = 0; q < R / 2; ++q) {
+        if constexpr (SH)  // rows 2q and 2q+1, each (low, high) comparison
+          rp[q] = ((cl[2 * q] * KP + ch[2 * q]) * 4u) |
+                  (((cl[2 * q + 1] * KP + ch[2 * q + 1]) * 4u) << 16);
+        else  // low comparison (rows 2q, 2q+1), high comparison
+          rp[q] = ((cl[2 * q] * KP + cl[2 * q + 1]) * 4u) |
+                  (((ch[2 * q] * KP + ch[2 * q + 1]) * 4u) << 16);
+      }
     }
     uint32_t H[R], E[R], KL[KN], KH[KN];
 #pragma unroll
@@ -328,10 +347,15 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         uint32_t sv[R];
 #pragma unroll
         for (int q = 0; q < R / 2; ++q) {
-          const uint32_t wlo = *(const uint32_t *)(tl + (rp[q] & 0xFFFFu));
-          const uint32_t whi = *(const uint32_t *)(th + (rp[q] >> 16));
-          sv[2 * q] = __builtin_amdgcn_perm(whi, wlo, 0x05040100u);
-          sv[2 * q + 1] = __builtin_amdgcn_perm(whi, wlo, 0x07060302u);
+          if constexpr (SH) {
+            sv[2 * q] = *(const uint32_t *)(tl + (rp[q] & 0xFFFFu));
+            sv[2 * q + 1] = *(const uint32_t *)(tl + (rp[q] >> 16));
+          } else {
+            const uint32_t wlo = *(const uint32_t *)(tl + (rp[q] & 0xFFFFu));
+            const uint32_t whi = *(const uint32_t *)(th + (rp[q] >> 16));
+            sv[2 * q] = __builtin_amdgcn_perm(whi, wlo, 0x05040100u);
+            sv[2 * q + 1] = __builtin_amdgcn_perm(whi, wlo, 0x07060302u);
+          }
         }
         // t2 of row r is diag + s; the old H of row r is consumed (as the
         // diagonal of row r + 1) before the new H of row r is written.
@@ -455,11 +479,12 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     if constexpr (PAIRKEY) {
       const int lim = kTagKeyMaxH - A.smax * (R << lg);
       if (__ballot(max(bh_lo, bh_hi) > lim) == 0) {
-        strip(rows, std::true_type{}, ss);
+        if (MODE == kModeRow && shared) strip(rows, std::true_type{}, std::true_type{}, ss);
+        else strip(rows, std::true_type{}, std::false_type{}, ss);
         return;
       }
     }
-    strip(rows, std::false_type{}, ss);
+    strip(rows, std::false_type{}, std::false_type{}, ss);
   };
   bool reached = false;
   for (int ss = 0; ss + 1 < nss; ++ss) {

@@ -9,18 +9,28 @@
 namespace ipuma {
 
 // Comparison p = q * ndb + d aligns database sequence d (A, the columns)
-// against query q (B, the rows).
+// against query q (B, the rows), or with query_cols the query as A and the
+// database sequence as B (the same score for a symmetric matrix): the pairs
+// of one query then share A, and the packed kernel's groups of one query read
+// both comparisons' scores of a row with one load.
 __global__ void k_search_pairs(int32_t nq, int32_t ndb, const int64_t *q_off,
                                const int32_t *q_len, const int64_t *db_off,
                                const int32_t *db_len, int64_t *a_off, int32_t *a_len,
-                               int64_t *b_off, int32_t *b_len) {
+                               int64_t *b_off, int32_t *b_len, int32_t query_cols) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (int64_t)nq * ndb) return;
   const int q = (int)(p / ndb), d = (int)(p - (int64_t)q * ndb);
-  a_off[p] = db_off[d];
-  a_len[p] = db_len[d];
-  b_off[p] = q_off[q];
-  b_len[p] = q_len[q];
+  if (query_cols) {
+    a_off[p] = q_off[q];
+    a_len[p] = q_len[q];
+    b_off[p] = db_off[d];
+    b_len[p] = db_len[d];
+  } else {
+    a_off[p] = db_off[d];
+    a_len[p] = db_len[d];
+    b_off[p] = q_off[q];
+    b_len[p] = q_len[q];
+  }
 }
 
 // Top k of one query per block: entries ordered by score (descending), then
@@ -155,11 +165,12 @@ static inline int nblocks(int64_t n, int bs) { return (int)((n + bs - 1) / bs); 
 hipError_t launch_search_pairs(int32_t nq, int32_t ndb, const int64_t *q_off,
                                const int32_t *q_len, const int64_t *db_off,
                                const int32_t *db_len, int64_t *a_off, int32_t *a_len,
-                               int64_t *b_off, int32_t *b_len, hipStream_t st) {
+                               int64_t *b_off, int32_t *b_len, int32_t query_cols,
+                               hipStream_t st) {
   const int64_t n = (int64_t)nq * ndb;
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_search_pairs, dim3(nblocks(n, 256)), dim3(256), 0, st, nq, ndb, q_off,
-                     q_len, db_off, db_len, a_off, a_len, b_off, b_len);
+                     q_len, db_off, db_len, a_off, a_len, b_off, b_len, query_cols);
   return hipGetLastError();
 }
 

@@ -259,6 +259,59 @@ def test_search_topk_emulated(emu_lib):
     eng.close()
 
 
+def test_search_shared_query_emulated(emu_lib):
+    """BLOSUM62 is symmetric, so the search puts each query on the columns and
+    the packed kernel's groups of one query read both comparisons' scores of a
+    row with one load: queries of 20-300 residues (two of equal length, whose
+    groups can mix them), database sequences with planted homologs, pipelines
+    of 2 and 4 lanes, every score against the oracle."""
+    import oracle as O
+    from ipuma_amd.workloads import arena_from_flat
+    I = emu_lib
+    rng = np.random.default_rng(12)
+    qlens = (20, 75, 75, 150, 300)
+    qs = [rng.integers(0, 20, n).astype(np.uint8) for n in qlens]
+    db = []
+    for i in range(90):
+        if i % 9 == 0:  # a homolog of query i % 5 at about 60% identity
+            q = qs[i % 5]
+            x = q.copy()
+            mut = rng.random(len(x)) < 0.4
+            x[mut] = rng.integers(0, 20, int(mut.sum()))
+            db.append(np.concatenate([rng.integers(0, 20, 5), x]).astype(np.uint8))
+        else:
+            db.append(rng.integers(0, 20, int(rng.integers(10, 200))).astype(np.uint8))
+    nq, ndb = len(qs), len(db)
+    lens = np.array([len(x) for x in qs + db], np.int64)
+    codes, offs = arena_from_flat(np.concatenate(qs + db), lens)
+    q_off, db_off = offs[:nq].copy(), offs[nq:].copy()
+    q_len = lens[:nq].astype(np.int32)
+    db_len = lens[nq:].astype(np.int32)
+    mat = O.select_matrix(O.SIM_BLOSUM62)
+    exp = np.zeros((nq, ndb), np.int64)
+    for qi in range(nq):
+        s, _, _ = O.sw_batch_codes(codes, db_off, db_len, np.full(ndb, q_off[qi]),
+                                   np.full(ndb, q_len[qi], np.int32), mat, -10, -1, True)
+        exp[qi] = s
+    assert exp.max() > 200  # the planted homologs
+    for lanes in (2, 4):
+        eng = I.SWAlgorithm(I.SWConfig(gapInit=-10, gapExtend=-1, similarity="blosum62",
+                                       datatype="aa"),
+                            I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+        eng.set_lanes_per_pair(lanes)
+        k = 10
+        scores = np.zeros(nq * ndb, np.int32)
+        ti = np.zeros(nq * k, np.int32)
+        ts = np.zeros(nq * k, np.int32)
+        eng.search_device(codes.ctypes.data, len(codes), q_off.ctypes.data, q_len.ctypes.data,
+                          db_off.ctypes.data, db_len.ctypes.data, k, scores.ctypes.data,
+                          ti.ctypes.data, ts.ctypes.data, nq=nq, ndb=ndb)
+        assert np.array_equal(scores.reshape(nq, ndb), exp), lanes
+        ri, rs = _topk_reference(exp, k)
+        assert np.array_equal(ti.reshape(nq, k), ri), lanes
+        eng.close()
+
+
 def test_band_groups_emulated(emu_lib):
     """Whole-wave groups of two or more super-strips (|B| > 1024) shared by the
     waves of a block, at the classifier's pipeline depth and with every pair
