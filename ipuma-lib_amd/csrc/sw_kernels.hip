@@ -1061,7 +1061,7 @@ template __global__ void k_fast<26, true>(FastArgs);
 // Re-runs int16-saturated comparisons and serves gap settings outside the
 // packed kernel's domain (and the int32 precision policy).
 // ---------------------------------------------------------------------------
-constexpr int kW32Rows = 8;
+constexpr int kW32Rows = 16;
 
 __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
   constexpr int R = kW32Rows;
@@ -1115,14 +1115,17 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
         boff[r] = i0 + r < n ? min((int)b[i0 + r], kp - 1) : 0;
         H[r] = 0;   // H(i, -1)
         E[r] = gI;  // E(i, -1)
-        rmax[r] = 0;
+        // a row past |B| never takes a maximum (nothing exceeds INT_MAX): the
+        // step needs no row check, the merge below skips such rows
+        rmax[r] = i0 + r < n ? 0 : INT32_MAX;
         rcol[r] = 0;
       }
       int out_h = 0, out_f = 0, col = 0;
       int hdiag = 0;  // H(i0 - 1, j - 1)
       // Lane 0's inputs at step t (column t: its A code and the boundary row's
       // H and F) come from blocks of 64 columns that every lane loads one
-      // column of, a block ahead, and lane 0 reads through a shuffle.  Lane 63
+      // column of, a block ahead; the wave reads lane t % 64's values with
+      // v_readlane (a uniform lane index: no LDS round trip).  Lane 63
       // overwrites column c of the boundary at step c + 63, after the load of
       // c's block (at step c - 64 or earlier).
       int a_cur, h_cur = 0, f_cur = 0, a_nxt, h_nxt = 0, f_nxt = 0;
@@ -1146,9 +1149,9 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
           load_block(t + 64, a_nxt, h_nxt, f_nxt);
         }
         const int sl = t & 63;
-        const int ac = __shfl(a_cur, sl);
-        const int bh = load_bnd ? __shfl(h_cur, sl) : 0;
-        const int bf = load_bnd ? __shfl(f_cur, sl) : 0;
+        const int ac = __builtin_amdgcn_readlane(a_cur, sl);
+        const int bh = load_bnd ? __builtin_amdgcn_readlane(h_cur, sl) : 0;
+        const int bf = load_bnd ? __builtin_amdgcn_readlane(f_cur, sl) : 0;
         const int up_h = (int)from_prev<kModeWave>((uint32_t)bh, (uint32_t)out_h, first);
         const int up_f = (int)from_prev<kModeWave>((uint32_t)bf, (uint32_t)out_f, first);
         col = (int)from_prev<kModeWave>((uint32_t)ac, (uint32_t)col, first);
@@ -1165,7 +1168,7 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
             E[r] = e;
             hup = h;
             fup = f;
-            if (i0 + r < n && h > rmax[r]) {
+            if (h > rmax[r]) {
               rmax[r] = h;
               rcol[r] = j;
             }
@@ -1182,7 +1185,7 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
       // rows in order, strict >: the lane's first maximum in B order so far
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        if (rmax[r] > best) {
+        if (i0 + r < n && rmax[r] > best) {
           best = rmax[r];
           bi = i0 + r;
           bj = rcol[r];

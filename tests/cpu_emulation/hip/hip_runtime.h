@@ -147,6 +147,8 @@ void launch(const char *name, K kernel, dim3 grid, dim3 block, Args... args) {
 #define __builtin_amdgcn_perm(a, b, s) emu::perm((a), (b), (s))
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_readfirstlane(v) emu::exchange_get((v), 0)
+// v_readlane: the lane index is uniform over the wave (an SGPR operand)
+#define __builtin_amdgcn_readlane(v, l) emu::exchange_get((v), (l) & 63)
 #define __builtin_amdgcn_wave_barrier() ((void)emu::exchange_get(0, 0))
 
 template <class T> inline T __shfl_xor(T v, int m) { return emu::exchange_get(v, emu::ctx.lane ^ m); }

@@ -706,7 +706,7 @@ def test_multidriver_shape_emulated(emu_lib, build_dir):
 
 def test_int32_wave_kernel_emulated(emu_lib):
     """The exact int32 kernel (one wave per comparison, a 64-lane pipeline of
-    8 rows per lane, the boundary row between super-strips in the wave's work
+    16 rows per lane, the boundary row between super-strips in the wave's work
     area): the int32 policy on pairs spanning up to two super-strips, gap
     settings outside the packed kernel's domain (a gap that extends for free,
     a positive gap-open value), and an int16-saturated pair over three
@@ -718,7 +718,7 @@ def test_int32_wave_kernel_emulated(emu_lib):
     for i in range(6):
         m = int(rng.integers(1, 120))
         a = "".join(rng.choice(list("ACGT"), m))
-        nb = int(rng.integers(520, 700)) if i % 3 == 0 else int(rng.integers(1, 120))
+        nb = int(rng.integers(1030, 1300)) if i % 3 == 0 else int(rng.integers(1, 120))
         b = "".join(rng.choice(list("ACGT"), max(0, nb - m // 2))) + a[: m // 2]
         A.append(a)
         B.append(b[:nb])
@@ -733,8 +733,8 @@ def test_int32_wave_kernel_emulated(emu_lib):
                                 I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1,
                                                 forwardOnly=fwd))
             eng.set_precision(prec)
-            if ma == 127:  # 300 x 127 > 32767 over three super-strips: saturates
-                A, B = A + ["A" * 300], B + ["A" * 1030]
+            if ma == 127:  # 300 x 127 > 32767, B over three super-strips: saturates
+                A, B = A + ["A" * 300], B + ["A" * 2060]
             exp = O.sw_batch(A, B, match=ma, mismatch=mi, ambiguity=mi, gap_init=gi,
                              gap_extend=ge, forward_only=fwd)
             got = eng.compare(A, B)
