@@ -264,7 +264,7 @@ def test_search_shared_query_emulated(emu_lib):
     the packed kernel's groups of one query read both comparisons' scores of a
     row with one load: queries of 20-300 residues (two of equal length, whose
     groups can mix them), database sequences with planted homologs, pipelines
-    of 2 and 4 lanes, every score against the oracle."""
+    of 4 lanes, every score against the oracle."""
     import oracle as O
     from ipuma_amd.workloads import arena_from_flat
     I = emu_lib
@@ -294,7 +294,7 @@ def test_search_shared_query_emulated(emu_lib):
                                    np.full(ndb, q_len[qi], np.int32), mat, -10, -1, True)
         exp[qi] = s
     assert exp.max() > 200  # the planted homologs
-    for lanes in (2, 4):
+    for lanes in (4,):
         eng = I.SWAlgorithm(I.SWConfig(gapInit=-10, gapExtend=-1, similarity="blosum62",
                                        datatype="aa"),
                             I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
@@ -382,18 +382,21 @@ def test_row_tag_keys_emulated(emu_lib):
             B.append("".join(rng.choice(list("ACGT"), int(rng.integers(1, 90)))))
         return A, B
 
-    cases = ((dict(gapInit=-6, gapExtend=-2, matchValue=7, mismatchValue=-4), batch(640),
-              (1, 4, 16)),
-             (dict(gapInit=-4, gapExtend=-1, matchValue=3, mismatchValue=-2), batch(1100), (64,)))
-    for cfg, (A, B), depths in cases:
+    # (configuration, batch, pipeline depths, forward only)
+    cases = ((dict(gapInit=-6, gapExtend=-2, matchValue=7, mismatchValue=-4), batch(600),
+              (1, 16), False),
+             (dict(gapInit=-4, gapExtend=-1, matchValue=3, mismatchValue=-2), batch(1040), (64,),
+              True))
+    for cfg, (A, B), depths, fwd in cases:
         exp = O.sw_batch(A, B, gap_init=cfg["gapInit"], gap_extend=cfg["gapExtend"],
                          match=cfg["matchValue"], mismatch=cfg["mismatchValue"],
-                         forward_only=False)
+                         forward_only=fwd)
         # the long pair passes the row-tag bound of its deepest super-strips
         assert int(np.max(exp[0])) > 4095 - cfg["matchValue"] * 16 * max(depths)
         for lanes in depths:
             eng = I.SWAlgorithm(I.SWConfig(**cfg),
-                                I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+                                I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1,
+                                                forwardOnly=fwd))
             eng.set_lanes_per_pair(lanes)
             got = eng.compare(A, B)
             for g, e, what in zip(got, exp, ("scores", "a_range", "b_range")):
