@@ -8,9 +8,15 @@ shallow) against the length of the longest-first queue's tail (a low cap makes
 more, shorter groups).  Where the balance lies depends on the batch's length
 distribution, so it is measured rather than modelled:
 
-* ``tune`` times a small grid of (rounds, step cap scale) on the caller's own
+* ``tune`` times a grid of (rounds, step cap scale) on the caller's own
   batch (one untimed pass, then the median of ``reps`` timed passes each),
-  applies the fastest and returns the table;
+  then refines below the fastest point in steps of 0.01, applies the fastest
+  and returns the table.  The time is a sawtooth in the scale: the queue runs
+  in rounds of about one group per wave, a larger cap makes every group
+  longer until the groups fit one round fewer, where the time drops (on the
+  protein batch 2.95 ms at scale 0.77 and 0.98, 3.10-3.33 ms just below
+  them).  The fastest point of a tooth lies just above its drop, so the
+  refinement walks down from the best grid point while the time improves;
 * ``TuneCache`` keeps the choice in a JSON file keyed by the device, the
   alphabet and a coarse signature of the batch (pair count, cells, length
   quantiles), so a later batch of the same kind reuses it without timing.
@@ -28,8 +34,10 @@ from typing import Callable, Iterable, Sequence
 
 import numpy as np
 
-# (rounds, step cap scale): the default first, then shallower and deeper caps
-CANDIDATES: tuple = ((1, 1.0), (1, 0.8), (1, 1.25), (1, 1.5), (1, 0.6), (2, 1.0))
+# (rounds, step cap scale): the default first, then a grid of caps (the cap
+# depends on scale / rounds only, so one round covers them)
+CANDIDATES: tuple = ((1, 1.0),) + tuple((1, round(0.6 + 0.05 * i, 2)) for i in range(13) if i != 8)
+REFINE_STEP = 0.01
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ipuma_amd", "autotune.json")
 
@@ -106,9 +114,12 @@ def apply(eng, rounds: int, scale: float):
 
 def tune(eng, run: Callable[[], None], key: str | None = None, cache: TuneCache | None = None,
          candidates: Iterable = CANDIDATES, reps: int = 3,
-         sync: Callable[[], None] | None = None) -> dict:
+         sync: Callable[[], None] | None = None, refine_step: float = REFINE_STEP,
+         refine_max: int = 4) -> dict:
     """Choose (rounds, step cap scale) for ``eng`` by timing ``run`` (one device
-    batch through ``eng``) under each candidate; apply the fastest and return
+    batch through ``eng``) under each candidate, then at up to ``refine_max``
+    scales below the fastest, ``refine_step`` apart, while the time improves
+    (``refine_step`` 0: no refinement); apply the fastest and return
     {"rounds", "scale", "ms", "table", "cached"}.  With a cache and a key
     already in it, the cached choice is applied without timing."""
     if cache is not None and key is not None:
@@ -118,7 +129,8 @@ def tune(eng, run: Callable[[], None], key: str | None = None, cache: TuneCache 
             return dict(hit, cached=True)
     sync = sync or eng.synchronize
     table = []
-    for rounds, scale in candidates:
+
+    def measure(rounds, scale):
         apply(eng, rounds, scale)
         run()  # untimed: first pass of a shape
         sync()
@@ -130,7 +142,23 @@ def tune(eng, run: Callable[[], None], key: str | None = None, cache: TuneCache 
             ms.append((time.perf_counter() - t0) * 1e3)
         table.append({"rounds": int(rounds), "scale": float(scale),
                       "ms": float(np.median(ms))})
+        return table[-1]
+
+    for rounds, scale in candidates:
+        measure(rounds, scale)
     best = min(table, key=lambda r: r["ms"])
+    if refine_step > 0:
+        tried = {(r["rounds"], round(r["scale"], 4)) for r in table}
+        cur = best
+        for i in range(1, refine_max + 1):
+            s = round(best["scale"] - i * refine_step, 4)
+            if s < 0.25 or (best["rounds"], s) in tried:
+                break
+            r = measure(best["rounds"], s)
+            if r["ms"] >= cur["ms"]:
+                break
+            cur = r
+        best = min(table, key=lambda r: r["ms"])
     apply(eng, best["rounds"], best["scale"])
     entry = {"rounds": best["rounds"], "scale": best["scale"], "ms": best["ms"], "table": table}
     if cache is not None and key is not None:

@@ -35,7 +35,8 @@ class FakeEngine:
 
     def run(self):
         self.passes += 1
-        self.clock += self.cost[(self.rounds, self.scale)]
+        c = self.cost
+        self.clock += c(self.rounds, self.scale) if callable(c) else c[(self.rounds, self.scale)]
 
 
 def test_signature_stable_and_coarse():
@@ -61,7 +62,7 @@ def test_tune_picks_the_fastest_and_caches(tmp_path, monkeypatch):
     # the timer reads the fake clock, so the choice depends only on `cost`
     monkeypatch.setattr(AT.time, "perf_counter", lambda: eng.clock / 1e3)
     cache = AT.TuneCache(str(tmp_path / "tune.json"))
-    r = AT.tune(eng, eng.run, key="k", cache=cache, reps=3)
+    r = AT.tune(eng, eng.run, key="k", cache=cache, reps=3, refine_step=0)
     assert (r["rounds"], r["scale"]) == best and not r["cached"]
     assert (eng.rounds, eng.scale) == best  # applied
     assert eng.passes == len(AT.CANDIDATES) * 4  # one untimed + three timed each
@@ -71,10 +72,28 @@ def test_tune_picks_the_fastest_and_caches(tmp_path, monkeypatch):
     r2 = AT.tune(eng2, eng2.run, key="k", cache=AT.TuneCache(str(tmp_path / "tune.json")))
     assert r2["cached"] and (eng2.rounds, eng2.scale) == best and eng2.passes == 0
     # another key is timed again
-    r3 = AT.tune(eng2, eng2.run, key="other", cache=cache)
+    r3 = AT.tune(eng2, eng2.run, key="other", cache=cache, refine_step=0)
     assert not r3["cached"] and eng2.passes > 0
     data = json.load(open(tmp_path / "tune.json"))
     assert set(data) == {"k", "other"}
+
+
+def test_tune_refines_down_to_the_drop(monkeypatch):
+    """A sawtooth like the measured one (time rises with the scale within a
+    tooth, drops where the groups fit one round fewer): the grid lands inside
+    a tooth, the refinement walks down to the point just above the drop and
+    stops at the first step that does not improve."""
+    def cost(rounds, scale):  # one tooth over the grid, dropping at 0.98
+        base = 3.0 + 2.0 * (scale - 0.98) if scale >= 0.98 - 1e-9 else 3.5 + 2.0 * (scale - 0.6)
+        return base + (0.5 if rounds > 1 else 0.0)
+    eng = FakeEngine(cost)
+    monkeypatch.setattr(AT.time, "perf_counter", lambda: eng.clock / 1e3)
+    r = AT.tune(eng, eng.run, reps=1)
+    assert r["rounds"] == 1 and abs(r["scale"] - 0.98) < 1e-9, r["table"]
+    assert (eng.rounds, eng.scale) == (1, r["scale"])
+    refined = r["table"][len(AT.CANDIDATES):]
+    # from the best grid point 1.0 down: 0.99, 0.98 improve, 0.97 (below the drop) does not
+    assert [round(t["scale"], 2) for t in refined] == [0.99, 0.98, 0.97]
 
 
 def test_cache_rejects_bad_entries(tmp_path):
@@ -106,7 +125,8 @@ def test_tuned_engine_stays_exact(gpu):
                            t["b_len"], *out)
 
     r = AT.tune(eng, run, reps=1)
-    assert (r["rounds"], r["scale"]) in AT.CANDIDATES
+    assert any(r["rounds"] == c[0] and abs(r["scale"] - c[1]) <= 4 * AT.REFINE_STEP + 1e-9
+               for c in AT.CANDIDATES)
     for rounds, scale in ((r["rounds"], r["scale"]), (1, 0.6), (2, 1.0)):
         AT.apply(eng, rounds, scale)
         got = T.device_run(eng, batch)
