@@ -39,6 +39,9 @@ constexpr int kNssBins = 64;
 constexpr int kMBins = 512;
 constexpr int kMBinsPerRows = 170;
 static_assert(3 * kMBinsPerRows <= kMBins, "m-bins of the three last-strip row counts");
+// Whole-wave groups of one super-strip: five row counts, m-bins of their own.
+constexpr int kMBinsPerRowsWave = 102;
+static_assert(5 * kMBinsPerRowsWave <= kMBins, "m-bins of the five whole-wave row counts");
 constexpr int kRuns = kNumClasses * kNssBins;
 constexpr int kNumBins = kRuns * kMBins;
 constexpr int kRunAlign = 128;
@@ -66,14 +69,24 @@ struct GroupDesc {
 };
 
 // Rows per lane the packed kernel runs the last super-strip of a group with:
-// the smallest of 8, 12 and kRows that covers `need`.
-__host__ __device__ inline int last_strip_rows(int need) {
+// the smallest of 8, 12 and kRows that covers `need`; whole-wave groups (one
+// super-strip of 64 lanes, where four rows per lane are 256 rows of B) take
+// the smallest even count from 8 up.
+__host__ __device__ inline int last_strip_rows(int need, bool wave = false) {
+  if (wave) return need <= 8 ? 8 : need >= kRows ? kRows : (need + 1) & ~1;
   return need <= 8 ? 8 : need <= 12 ? 12 : kRows;
 }
+// Rows class of a comparison's last super-strip in the bucketing (pairs of
+// one class share groups): 8, 12, 16 rows per lane, or for whole-wave groups
+// of one super-strip 8, 10, 12, 14, 16.
+__host__ __device__ inline int rows_class(int need, bool wave) {
+  if (wave) return need <= 8 ? 0 : (min(need, kRows) - 7) / 2;
+  return need <= 8 ? 0 : need <= 12 ? 1 : 2;
+}
 // Cost of one step with `rows` rows per lane, in 32nds of a kRows step (the
-// step bodies take about 135, 187 and 240 VALU instructions).
+// step bodies take about 135, 187 and 240 VALU instructions for 8, 12, 16).
 __host__ __device__ inline int step_weight32(int rows) {
-  return rows <= 8 ? 18 : rows <= 12 ? 25 : 32;
+  return rows <= 8 ? 18 : rows <= 10 ? 21 : rows <= 12 ? 25 : rows <= 14 ? 28 : 32;
 }
 
 // Control block of the device bucketing (one per batch, zeroed before use).

@@ -199,7 +199,8 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   // of a group the 8 or 12 that cover its rows (lane k then owns rows
   // ss * L * kRows + k * R .. + R).
   // weight of the group's last super-strip (its rows per lane)
-  const int wlast = step_weight32(PAIRKEY ? last_strip_rows(gd.last_rows) : kRows);
+  const int wlast =
+      step_weight32(PAIRKEY ? last_strip_rows(gd.last_rows, MODE == kModeWave) : kRows);
 
   // Column table of the group, behind the boundary row in the wave's scratch:
   // entry (t, g) packs the LDS table offsets (in words) of the A codes at
@@ -248,19 +249,27 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     const int row0 = (ss << lg) * kRows + k * R;
     uint32_t rp[R / 2];  // row-pair table offsets: low half low comparison, high half high comparison
     {
-      uint32_t wl[R / 4], wh[R / 4];
-      const uint32_t *bl = (const uint32_t *)(b_lo + row0);
-      const uint32_t *bh = (const uint32_t *)(b_hi + row0);
-#pragma unroll
-      for (int q = 0; q < R / 4; ++q) {
-        wl[q] = (row0 + 4 * q < n_lo) ? bl[q] : 0u;
-        wh[q] = (row0 + 4 * q < n_hi) ? bh[q] : 0u;
-      }
       uint32_t cl[R], ch[R];
+      if constexpr (R % 4 == 0) {  // row0 is a multiple of 4: word loads
+        uint32_t wl[R / 4], wh[R / 4];
+        const uint32_t *bl = (const uint32_t *)(b_lo + row0);
+        const uint32_t *bh = (const uint32_t *)(b_hi + row0);
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        cl[r] = (row0 + r < n_lo) ? min((wl[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
-        ch[r] = (row0 + r < n_hi) ? min((wh[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
+        for (int q = 0; q < R / 4; ++q) {
+          wl[q] = (row0 + 4 * q < n_lo) ? bl[q] : 0u;
+          wh[q] = (row0 + 4 * q < n_hi) ? bh[q] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          cl[r] = (row0 + r < n_lo) ? min((wl[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
+          ch[r] = (row0 + r < n_hi) ? min((wh[r >> 2] >> (8 * (r & 3))) & 0xffu, PADC) : PADC;
+        }
+      } else {  // 10 or 14 rows per lane: row0 = k * R is not word-aligned
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          cl[r] = (row0 + r < n_lo) ? min((uint32_t)b_lo[row0 + r], PADC) : PADC;
+          ch[r] = (row0 + r < n_hi) ? min((uint32_t)b_hi[row0 + r], PADC) : PADC;
+        }
       }
 #pragma unroll
       for (int q = 0; q < R / 2; ++q) {
@@ -503,9 +512,11 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   }
   if (nss > 0 && !reached) {
     if constexpr (PAIRKEY) {
-      const int rl = last_strip_rows(gd.last_rows);
+      const int rl = last_strip_rows(gd.last_rows, MODE == kModeWave);
       if (rl == 8) run_strip(std::integral_constant<int, 8>{}, nss - 1);
       else if (rl == 12) run_strip(std::integral_constant<int, 12>{}, nss - 1);
+      else if (MODE == kModeWave && rl == 10) run_strip(std::integral_constant<int, 10>{}, nss - 1);
+      else if (MODE == kModeWave && rl == 14) run_strip(std::integral_constant<int, 14>{}, nss - 1);
       else run_strip(Full{}, nss - 1);
     } else {
       run_strip(Full{}, nss - 1);
@@ -1279,6 +1290,11 @@ __device__ __forceinline__ int nss_bin_of(int nss) {
 __device__ __forceinline__ int mbin_of(int m) {
   return m <= 2048 ? (m + 15) / 16 : min(kMBinsPerRows - 1, 128 + (m - 2048 + 255) / 256);
 }
+// |A| bin of a whole-wave pair of one super-strip (five row counts share the
+// run's bins): 16 columns wide up to 1024, 64 wide up to 3328, then one bin.
+__device__ __forceinline__ int mbin_wave_of(int m) {
+  return m <= 1024 ? (m + 15) / 16 : min(kMBinsPerRowsWave - 1, 64 + (m - 1024 + 63) / 64);
+}
 
 // Device-scope atomics on one address serialize across the XCDs, so the
 // batch-wide sums below are reduced per block first: one atomic per block.
@@ -1396,11 +1412,14 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
       const int nb_key = target ? min(nb, 2 * max(target[p], 0) + 16) : nb;
       const int nss = (nb_key + L * kRows - 1) / (L * kRows);
       // rows per lane the last super-strip of |B| needs (a group runs it with
-      // 8, 12 or 16, by its longest B): pairs that need fewer share groups
+      // 8, 12 or 16, or a whole-wave group of one super-strip with an even
+      // count from 8, by its longest B): pairs that need fewer share groups
       const int nss_b = (nb + L * kRows - 1) / (L * kRows);
       const int last = (nb - (nss_b - 1) * L * kRows + L - 1) / L;
-      const int rlc = last <= 8 ? 0 : last <= 12 ? 1 : 2;
-      bin = (cls * kNssBins + nss_bin_of(nss)) * kMBins + rlc * kMBinsPerRows + mbin_of(m);
+      const bool wave1 = cls == kNumClasses - 1 && nss_b == 1;
+      const int rlc = rows_class(last, wave1);
+      bin = (cls * kNssBins + nss_bin_of(nss)) * kMBins +
+            (wave1 ? rlc * kMBinsPerRowsWave + mbin_wave_of(m) : rlc * kMBinsPerRows + mbin_of(m));
       bin_of[p] = bin;
       atomicAdd(&s_count[cls], 1);
     }
@@ -1574,7 +1593,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, co
       d.last_rows = d.nss > 0 ? (nbmax - (d.nss - 1) * L * kRows + L - 1) / L : kRows;
       gtmp[gi] = d;
       const bool band_g = c == kNumClasses - 1 && d.nss >= 2;  // band strips run kRows rows
-      const int rows_last = band_g || d.nss == 0 ? kRows : last_strip_rows(d.last_rows);
+      const int rows_last =
+          band_g || d.nss == 0 ? kRows : last_strip_rows(d.last_rows, c == kNumClasses - 1);
       pad += (unsigned long long)128 * (unsigned long long)mpad *
              (unsigned long long)(d.nss > 0 ? (d.nss - 1) * kRows + rows_last : 0);
       // longest groups first over the whole batch (a wave's time is about its

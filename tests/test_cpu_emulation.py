@@ -134,15 +134,17 @@ def test_shape_knobs_emulated(emu_lib):
 
 def test_last_strip_rows_emulated(emu_lib):
     """Pipelines of 2, 4 and 16 lanes whose last super-strip needs 1-16 rows per
-    lane (run with 8, 12 or 16), forward and with starts (whose pass stops
-    early), against the oracle."""
+    lane (run with 8, 12 or 16), and whole-wave groups of one super-strip
+    needing 7-16 rows per lane (run with 8, 10, 12, 14 or 16), forward and with
+    starts (whose pass stops early), against the oracle."""
     import oracle as O
     I = emu_lib
     rng = np.random.default_rng(43)
-    for lanes in (2, 4, 16):
+    for lanes in (2, 4, 16, 64):
         A, B = [], []
-        for need in (1, 3, 8, 9, 12, 13, 16):
-            nb = 16 * lanes + lanes * (need - 1) + 1  # one full super-strip, then `need` rows per lane
+        for need in ((1, 3, 8, 9, 12, 13, 16) if lanes < 64 else (7, 9, 10, 11, 13, 14, 15, 16)):
+            # one full super-strip, then `need` rows per lane (whole-wave: one super-strip)
+            nb = (16 * lanes if lanes < 64 else 0) + lanes * (need - 1) + 1
             a = "".join(rng.choice(list("ACGT"), int(rng.integers(20, 60))))
             b = "".join(rng.choice(list("ACGT"), nb - len(a) // 2)) + a[: len(a) // 2]
             A.append(a)
