@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--with-starts", action="store_true", help="time the reverse pass too")
     ap.add_argument("--rounds", type=int, default=0,
                     help="groups per resident wave for the bucketing (0 = engine default)")
+    ap.add_argument("--cap-scale", type=float, default=0.0,
+                    help="step cap scale of the protein batch's bucketing (0 = engine default or "
+                         "the autotuner's choice; set: no autotuning of that batch)")
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="time a small grid of bucketing shapes (rounds x step cap scale) on "
                          "each batch before its timed region and keep the fastest (off with "
@@ -578,7 +581,10 @@ def main():
     batch = W.config1(n=args.pairs)
     log(f"generated {len(batch['a_len'])} pairs, {batch['cells']:.3e} cells in {time.time()-t:.1f}s")
     db = DeviceBatch(batch, forward_only=not args.with_starts, rounds=args.rounds)
-    tuned = db.tune("blosum62") if (args.autotune == "on" and not args.rounds) else None
+    if args.cap_scale:
+        db.eng.set_step_cap_scale(args.cap_scale)
+    tuned = (db.tune("blosum62")
+             if (args.autotune == "on" and not args.rounds and not args.cap_scale) else None)
     ms_step, fast_avg, st = db.timed(args.steps, args.warmup)
     cells = batch["cells"]
     gcups = cells / ms_step / 1e6
@@ -662,6 +668,7 @@ def main():
                      "positions",
          "pairs": n, "cells": cells, "starts": bool(args.with_starts),
          "work_rounds": args.rounds or "default",
+         "step_cap_scale": args.cap_scale or None,
          "autotune": ({"rounds": tuned["rounds"], "step_cap_scale": tuned["scale"],
                        "candidates": len(tuned["table"])} if tuned else None)},
         "int16x2 (int32 re-run on saturation)")
