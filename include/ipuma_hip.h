@@ -308,7 +308,7 @@ typedef struct ipuma_batch_stats {
   int32_t lanes_per_pair_max;
   int32_t rows_per_lane;
   int32_t band_groups;     /* whole-wave groups shared by the waves of a block */
-  int32_t reserved;
+  int32_t head_groups;     /* longest other groups, started with the band groups */
   double fast_kernel_ms;   /* packed kernel, HIP events on its stream (timing on) */
   double batch_ms;         /* whole device batch, same events (timing on) */
 } ipuma_batch_stats;

@@ -54,6 +54,9 @@ static_assert(kCostBins >= kCostBinsPerClass, "cost bins");
 // super-strips is shared by the waves of one block, one super-strip per wave
 // at a time; a block keeps up to one band group per wave in flight, each with
 // its own boundary row of kBandScratchWords words.
+// Groups outside the band groups whose steps reach a quarter of the step cap
+// are head groups: they start with the band groups instead of after them.
+constexpr int kHeadCapDiv = 4;
 constexpr int kBandSlots = kFastThreads / 64;
 constexpr int kMaxStrips = 64;  // super-strips of 1,024 rows cover 65,535 rows
 // Saturation value of the packed int16 path.
@@ -103,7 +106,9 @@ struct BucketCtl {
   int32_t queue;       // next per-wave group of the packed kernel's work queue
   int32_t nband;       // band groups: the first nband entries of the group order
   int32_t band_queue;  // next band group (taken by a whole block)
-  int32_t pad0;
+  int32_t nhead;       // head groups: the nhead entries after the band groups
+  int32_t head_queue;  // next head group
+  int32_t head_cost;   // steps from which a group that is not a band group is a head group
   unsigned long long total_cells;
   unsigned long long padded_cells;
 };

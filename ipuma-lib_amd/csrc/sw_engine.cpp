@@ -1287,6 +1287,7 @@ static int stats_locked(ipuma_engine *e, ipuma_batch_stats *out) {
   out->lanes_per_pair_max = lmax;
   out->rows_per_lane = kRows;
   out->band_groups = c.nband;
+  out->head_groups = c.nband > 0 ? std::min(c.nhead, c.ngroups - c.nband) : 0;
   if (e->ev_recorded) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, e->ev[1], e->ev[2]));
@@ -1483,11 +1484,12 @@ static void log_batch(ipuma_engine *e, const char *entry, double host_ms, int st
   int k = std::snprintf(line, sizeof(line),
                         "IPUSWLOG{\"tag\":\"batch_perf\",\"entry\":\"%s\",\"status\":%d,"
                         "\"pairs\":%lld,\"cells\":%lld,\"padded_cells\":%lld,\"groups\":%lld,"
-                        "\"rerun_pairs\":%lld,\"band_groups\":%d,\"kernel_ms\":%.4f,"
+                        "\"rerun_pairs\":%lld,\"band_groups\":%d,\"head_groups\":%d,\"kernel_ms\":%.4f,"
                         "\"batch_ms\":%.4f,\"gcups_inner\":%.2f,\"gcups_batch\":%.2f",
                         entry, status, (long long)st.pairs, (long long)st.cells,
                         (long long)st.padded_cells, (long long)st.groups,
-                        (long long)st.rerun_pairs, st.band_groups, st.fast_kernel_ms, st.batch_ms,
+                        (long long)st.rerun_pairs, st.band_groups, st.head_groups, st.fast_kernel_ms,
+                        st.batch_ms,
                         st.fast_kernel_ms > 0 ? cells / st.fast_kernel_ms / 1e6 : 0.0,
                         st.batch_ms > 0 ? cells / st.batch_ms / 1e6 : 0.0);
   if (host_ms > 0 && k > 0 && k < (int)sizeof(line))

@@ -592,9 +592,10 @@ struct BandState {
   int32_t free_mask;              // slots not holding a group
 };
 static_assert(kBandSlots < 16 && kMaxStrips < 255, "band ticket layout");
-// Band groups of at least this many super-strips run their strips at raised
-// wave priority.
-constexpr int kBandPrioStrips = 8;
+// Steps by which super-strip s + 1 of a band group trails super-strip s: it
+// reads boundary column j at its step j - 1, which strip s writes at its step
+// j + 63 and publishes at most 16 steps later.
+constexpr int kBandLag = 76;
 
 // Loads of LDS words other waves update are read back through readfirstlane:
 // the wave's control flow depends on them, so it must see one value.
@@ -628,7 +629,7 @@ __device__ __forceinline__ int band_wait(int *prog, int need) {
 // of run_group, with the boundary row shared through the slot's buffer.
 template <int KP, bool PAIRKEY>
 __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
-                           int sl, int s, int lane) {
+                           int sl, int s, int lane, const SimdShare &share) {
   constexpr int R = kRows;
   constexpr uint32_t PADC = KP - 1;
   constexpr uint32_t COLSTRIDE = KP * KP * 4;
@@ -717,6 +718,15 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
   }
   const int T = mpad + 63;
   for (int t0 = 0; t0 < T; t0 += 4) {
+    if ((t0 & 7) == 0) {
+      // The group's remaining critical path through this strip, in 32nds of
+      // a kRows step like the other groups' work left: this strip's steps,
+      // the lag of each strip after it, and one more strip's length while
+      // some of the group's strips wait for a wave.
+      const int tk = lds_load_acq(&S.ticket);
+      const bool waiting = (tk & 0xFF) < nss;
+      share_simd(share, ((T - t0) + (nss - 1 - s) * kBandLag + (waiting ? T : 0)) * 32, lane);
+    }
     // columns t0 + 1 .. t0 + 4 of the boundary row are read below
     if (load_bnd) {
       const int need = min(mpad, t0 + 5);
@@ -932,7 +942,7 @@ __device__ __forceinline__ void band_load(const FastArgs &A, BandState &B, int c
 // until no band group is left.
 template <int KP>
 __device__ __forceinline__ void band_phase(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
-                           int lane, int nband) {
+                           int lane, int nband, const SimdShare &share) {
   for (;;) {
     const int cur = lds_load_acq(&B.cur);
     if (cur < 0) return;
@@ -947,14 +957,14 @@ __device__ __forceinline__ void band_phase(const FastArgs &A, const uint32_t *__
     if (nx <= nss) {
       if (lds_cas_lane0(&S.ticket, t, t + 1, lane) != t) continue;  // another wave took it
       if (nx < nss) {
-        // The longest band groups bound the launch: their strips issue ahead
-        // of the other waves on the SIMD (the other waves fill the gaps).
-        if (nss >= kBandPrioStrips) __builtin_amdgcn_s_setprio(2);
+        // The strip ranks against the other waves on its SIMD by the work
+        // left on its path (the longest path issues first, the others fill
+        // its gaps), as every group of the main queue does.
         if (__builtin_amdgcn_readfirstlane(S.gd.mpad) <= kPairKeyMaxM)
-          band_strip<KP, true>(A, tab, B, sl, nx, lane);
+          band_strip<KP, true>(A, tab, B, sl, nx, lane, share);
         else
-          band_strip<KP, false>(A, tab, B, sl, nx, lane);
-        __builtin_amdgcn_s_setprio(0);
+          band_strip<KP, false>(A, tab, B, sl, nx, lane, share);
+        share_simd(share, -1, lane);
       } else {
         band_load(A, B, cur, lane, nband);
       }
@@ -1006,8 +1016,6 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
              ((uint32_t)(uint16_t)A.table[a * KP + b2] << 16);
   }
   __syncthreads();
-  // band groups first (they are the longest), one block each, then the rest
-  if (nband > 0) band_phase<KP>(A, tab, band, lane, nband);
   const int wslot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int wave = (int)blockIdx.x * WPB + wslot;
   uint32_t *wscr = A.scratch + (int64_t)wave * A.wave_words;
@@ -1019,8 +1027,24 @@ __global__ __launch_bounds__(kFastThreads) void k_fast(FastArgs A) {
     }
   share.p1 = __builtin_amdgcn_readfirstlane(share.p1);
   share.p2 = __builtin_amdgcn_readfirstlane(share.p2);
-  for (int gi = next_group(&A.ctl->queue, nband, ng, lane); gi >= 0;
-       gi = next_group(&A.ctl->queue, nband, ng, lane)) {
+  // Band groups first (they are the longest), one block each, then the head
+  // groups (the longest of the rest), then the rest.  When there are band
+  // groups, the first wave of each SIMD takes one head group before the band
+  // phase instead: behind it the head groups would end the launch, and this
+  // way they start spread one per SIMD.
+  const int nhead = nband > 0 ? min(A.ctl->nhead, ng - nband) : 0;
+  bool head_left = nhead > 0;
+  int gi = -1;
+  if (head_left && wslot < share.p1 && wslot < share.p2)
+    gi = next_group(&A.ctl->head_queue, nband, nband + nhead, lane);
+  if (gi < 0 && nband > 0) band_phase<KP>(A, tab, band, lane, nband, share);
+  for (;; gi = -1) {
+    if (gi < 0 && head_left) {
+      gi = next_group(&A.ctl->head_queue, nband, nband + nhead, lane);
+      head_left = gi >= 0;
+    }
+    if (gi < 0) gi = next_group(&A.ctl->queue, nband + nhead, ng, lane);
+    if (gi < 0) break;
     const GroupDesc gd = A.groups[gi];
     // a descriptor outside what the bucketing produces (class, boundary row
     // and column table within the wave's scratch, rows <= kMaxSeqLen) is
@@ -1363,6 +1387,11 @@ __global__ __launch_bounds__(kClsThreads) void k_classify(int32_t n, const int32
   }
   __syncthreads();
   const int p = blockIdx.x * blockDim.x + tid;
+  if (p == 0) {
+    const int64_t cap = max((int64_t)1, (int64_t)(ctl->total_cells /
+                                                  (unsigned long long)(kRows * target_slots)));
+    ctl->head_cost = (int32_t)min(cap / kHeadCapDiv, (int64_t)0x7FFFFFFF);
+  }
   int bin = -1;
   if (p < n) {
     const int m = a_len[p], nb = b_len[p];
@@ -1551,10 +1580,15 @@ __global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, co
                                                           int32_t *grank, int32_t *costhist) {
   __shared__ int32_t s_cost[kCostBins];
   __shared__ unsigned long long s_pad[kGroupThreads / 64];
-  __shared__ int32_t s_nband;
+  __shared__ int32_t s_nband, s_nhead;
   static_assert(kCostBins >= 2 * kCostBinsPerClass, "band and other cost bins");
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (tid == 0) s_nband = 0;
+  if (tid == 0) {
+    s_nband = 0;
+    s_nhead = 0;
+  }
+  // groups in this cost bin or above that are not band groups are head groups
+  const int head_bin = cost_bin(ctl->head_cost);
   for (int i = tid; i < kCostBins; i += blockDim.x) s_cost[i] = 0;
   __syncthreads();
   const int ng = min(max_groups, ctl->ngroups);
@@ -1604,11 +1638,12 @@ __global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, co
       const bool band = c == kNumClasses - 1 && d.nss >= 2;
       const int64_t strips32 =
           d.nss > 0 ? (int64_t)(d.nss - 1) * 32 + step_weight32(rows_last) : 0;
-      const int cb = (band ? 0 : kCostBinsPerClass) + kCostBinsPerClass - 1 -
-                     cost_bin(strips32 * (mpad + L - 1) / 32);
+      const int cbin = cost_bin(strips32 * (mpad + L - 1) / 32);
+      const int cb = (band ? 0 : kCostBinsPerClass) + kCostBinsPerClass - 1 - cbin;
       gbin[gi] = cb;
       grank[gi] = atomicAdd(&s_cost[cb], 1);
       if (band) atomicAdd(&s_nband, 1);
+      else if (cbin >= head_bin) atomicAdd(&s_nhead, 1);
     }
   }
   if (lane == 0) s_pad[w] = pad;
@@ -1622,6 +1657,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_groups(int32_t max_groups, co
     for (int i = 0; i < nw; ++i) t += s_pad[i];
     if (t) atomicAdd(&ctl->padded_cells, t);
     if (s_nband) atomicAdd(&ctl->nband, s_nband);
+    if (s_nhead) atomicAdd(&ctl->nhead, s_nhead);
   }
   __syncthreads();
   for (int gi = (int)blockIdx.x * nw + w; gi < ng; gi += stride)

@@ -59,11 +59,11 @@ class BatchStats(ctypes.Structure):
                 ("padded_cells", ctypes.c_int64), ("groups", ctypes.c_int64),
                 ("rerun_pairs", ctypes.c_int64), ("lanes_per_pair_max", ctypes.c_int32),
                 ("rows_per_lane", ctypes.c_int32), ("band_groups", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("fast_kernel_ms", ctypes.c_double),
+                ("head_groups", ctypes.c_int32), ("fast_kernel_ms", ctypes.c_double),
                 ("batch_ms", ctypes.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class _ServerConfig(ctypes.Structure):
