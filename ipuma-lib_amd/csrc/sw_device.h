@@ -59,8 +59,11 @@ static_assert(kCostBins >= kCostBinsPerClass, "cost bins");
 constexpr int kHeadCapDiv = 4;
 constexpr int kBandSlots = kFastThreads / 64;
 constexpr int kMaxStrips = 64;  // super-strips of 1,024 rows cover 65,535 rows
-// Saturation value of the packed int16 path.
-constexpr int32_t kSat16 = 32767;
+// Saturation of the packed int16 path: the packed kernel holds its DP values
+// plus kBias16, as positive normal fp16 bit patterns below 0x7C00, and a
+// comparison whose maximum reaches kSat16 is re-run in int32.
+constexpr int32_t kBias16 = 1024;
+constexpr int32_t kSat16 = 0x7C00 - kBias16;
 
 // Group descriptor: one wave's work in the packed kernel.
 struct GroupDesc {

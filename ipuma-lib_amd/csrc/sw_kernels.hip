@@ -39,6 +39,23 @@ __device__ __forceinline__ s16x2 S2(uint32_t v) { return __builtin_bit_cast(s16x
 __device__ __forceinline__ uint32_t W(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ uint32_t W(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 
+// The larger of three packed 16-bit values per half, in one v_pk_maximum3_f16
+// (the compiler fuses this nesting of fp16 maximum).  The DP's values are
+// biased by kBias16 so that every H, E and F is a positive normal fp16 bit
+// pattern (0x0400 .. 0x7BFF), and the diagonal candidate is one too, a
+// positive subnormal (below zero by at most 128: the scores are 8-bit), or a
+// negative finite pattern (the pad score's -32768 added): on those the fp16
+// order is the integer order, and a pattern at 0x7C00 or above (an overflow,
+// Inf or NaN) yields one at 0x7C00 or above, which flags the comparison.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ f16x2 F2(uint32_t v) { return __builtin_bit_cast(f16x2, v); }
+__device__ __forceinline__ uint32_t max3h(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_bit_cast(uint32_t,
+                            __builtin_elementwise_maximum(F2(a), __builtin_elementwise_maximum(F2(b), F2(c))));
+}
+// kBias16 in both halves: H = 0 (and the floor of E and F)
+constexpr uint32_t kFloor2 = (uint32_t)kBias16 * 0x10001u;
+
 // Pipeline modes of the packed kernel: one lane per comparison (L = 1, no
 // cross-lane moves), 2..16 lanes (DPP row_shr:1 inside rows of 16 lanes), and
 // the whole wave (L = 64, DPP wave_shr:1).
@@ -87,9 +104,11 @@ __device__ __forceinline__ uint32_t from_prev(uint32_t own, uint32_t v, bool fir
 // Arithmetic is the reference recurrence with E and F clamped at 0:
 // H = max(diag + s, E', F') with E' = max(E, 0), F' = max(F, 0) is unchanged
 // because H is clamped at 0, and E'/F' follow E' <- max(H - o, E' - x, 0) when
-// o = -(gI + gE) >= 0 and x = -gE >= 0 (unsigned saturating subtraction).  The
-// add of the substitution score saturates at 32767; a comparison whose
-// maximum reaches 32767 is re-run exactly in int32.
+// o = -(gI + gE) >= 0 and x = -gE >= 0.  Every value is held plus kBias16
+// (1024): H = max3(diag + s, E', F') and E' = max3(H - o, E' - x, 1024) are
+// one v_pk_maximum3_f16 each after the unsigned saturating subtractions (see
+// max3h), and a maximum of at least kSat16 (30720: a biased H at 0x7C00) is
+// re-run exactly in int32.
 //
 // Padding (columns past |A|, rows past |B|, pipeline fill and drain) scores
 // the pad code at -32768, so padded cells only carry gap chains of real cells
@@ -284,20 +303,20 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     uint32_t H[R], E[R], KL[KN], KH[KN];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      H[r] = 0u;
-      E[r] = 0u;
+      H[r] = kFloor2;
+      E[r] = kFloor2;
     }
 #pragma unroll
     for (int q = 0; q < KN; ++q) {
       KL[q] = 0u;
       KH[q] = 0u;
     }
-    uint32_t out_h = 0u, out_f = 0u, hdiag = 0u;
+    uint32_t out_h = kFloor2, out_f = kFloor2, hdiag = kFloor2;
     uint32_t colw = PADC * KK * 0x10001u;  // the column's table offsets: low | high << 16
     const bool load_bnd = ss > 0;
     const bool store_bnd = ss + 1 < nss;
 
-    uint2 nb = make_uint2(0u, 0u);
+    uint2 nb = make_uint2(kFloor2, kFloor2);
     if (first && load_bnd && mpad > 0) nb = *(const uint2 *)(wscr + (size_t)g * 2);
     uint32_t ncw = 0u;
     uint32_t na_lo = 0u, na_hi = 0u;  // one-lane pipelines: the next four A codes
@@ -329,7 +348,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         if (first && load_bnd && t + 1 < mpad)
           nb = *(const uint2 *)((const char *)wscr + (uint32_t)(((t + 1) * NG + g) * 8));
         else
-          nb = make_uint2(0u, 0u);
+          nb = make_uint2(kFloor2, kFloor2);
         const uint32_t rh = from_prev<MODE>(cb.x, out_h, first);
         const uint32_t rf = from_prev<MODE>(cb.y, out_f, first);
         uint32_t col_lo, col_hi;  // byte offsets of the column's A-code rows of the table
@@ -384,18 +403,19 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         for (int r = 0; r < R; ++r) {
           s16x2 t2n = t2;
           if (r + 1 < R) t2n = __builtin_elementwise_add_sat(S2(H[r]), S2(sv[r + 1]));
-          const uint32_t hn =
-              W(__builtin_elementwise_max(__builtin_elementwise_max(t2, S2(E[r])), S2(f)));
+          const uint32_t hn = max3h(W(t2), E[r], f);
           H[r] = hn;
-          const u16x2 ho = __builtin_elementwise_sub_sat(U2(hn), U2(gap_o));
-          E[r] = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))));
-          f = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(f), U2(gap_x))));
+          const uint32_t ho = W(__builtin_elementwise_sub_sat(U2(hn), U2(gap_o)));
+          E[r] = max3h(ho, W(__builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))), kFloor2);
+          f = max3h(ho, W(__builtin_elementwise_sub_sat(U2(f), U2(gap_x))), kFloor2);
           if constexpr (TAG) {
             // H < 4096 in both halves: H * 16 + (15 - r) in 16 bits orders
-            // (max H, min row) within the step.  One v_pk_mad_u16 per row:
-            // the multiplier is opaque to the compiler (a shift and an or
-            // would take two instructions, and the tag constants registers)
-            const u16x2 kr = U2(hn) * U2(tag_mul) + (unsigned short)(15 - r);
+            // (max H, min row) within the step; from the biased H that is
+            // H' * 16 + (15 - r - 16 * kBias16) modulo 2^16.  One
+            // v_pk_mad_u16 per row: the multiplier is opaque to the compiler
+            // (a shift and an or would take two instructions, and the tag
+            // constants registers)
+            const u16x2 kr = U2(hn) * U2(tag_mul) + (unsigned short)(15 - r - 16 * kBias16);
             // two running maxima (even and odd rows) keep the chain short
             if (r < 2) tk2[r] = kr;
             else tk2[r & 1] = __builtin_elementwise_max(tk2[r & 1], kr);
@@ -447,7 +467,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
     }
 #pragma unroll
     for (int q = 0; q < KN; ++q) {
-      const int hl = (int)(KL[q] >> 16);
+      const int hl = (int)(KL[q] >> 16) - kBias16;
       if (hl > bh_lo) {
         bh_lo = hl;
         if constexpr (PAIRKEY) {
@@ -458,7 +478,7 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
           bj_lo = 0xFFFF - (int)(KL[q] & 0xFFFFu);
         }
       }
-      const int hh = (int)(KH[q] >> 16);
+      const int hh = (int)(KH[q] >> 16) - kBias16;
       if (hh > bh_hi) {
         bh_hi = hh;
         if constexpr (PAIRKEY) {
@@ -690,15 +710,15 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
   uint32_t H[R], E[R], KL[KN], KH[KN];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    H[r] = 0u;
-    E[r] = 0u;
+    H[r] = kFloor2;
+    E[r] = kFloor2;
   }
 #pragma unroll
   for (int q = 0; q < KN; ++q) {
     KL[q] = 0u;
     KH[q] = 0u;
   }
-  uint32_t out_h = 0u, out_f = 0u, hdiag = 0u;
+  uint32_t out_h = kFloor2, out_f = kFloor2, hdiag = kFloor2;
   uint32_t col_lo = PADC * COLSTRIDE, col_hi = PADC * COLSTRIDE;
   const bool load_bnd = s > 0;
   const bool store_bnd = s + 1 < nss;
@@ -706,7 +726,7 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
   int *const my_prog = &S.prog[s];
   int avail = 0;
 
-  uint2 nb = make_uint2(0u, 0u);
+  uint2 nb = make_uint2(kFloor2, kFloor2);
   if (load_bnd && mpad > 0) {
     avail = band_wait(prev_prog, 1);
     if (first) nb = *(const uint2 *)bscr;
@@ -745,7 +765,7 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
       if (first && load_bnd && t + 1 < mpad)
         nb = *(const uint2 *)(bscr + (size_t)(t + 1) * 2);
       else
-        nb = make_uint2(0u, 0u);
+        nb = make_uint2(kFloor2, kFloor2);
       const uint32_t ac_lo = (t < m_lo) ? min((wa_lo >> (8 * u)) & 0xffu, PADC) : PADC;
       const uint32_t ac_hi = (t < m_hi) ? min((wa_hi >> (8 * u)) & 0xffu, PADC) : PADC;
       const uint32_t rh = from_prev<kModeWave>(cb.x, out_h, first);
@@ -777,12 +797,11 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
       for (int r = 0; r < R; ++r) {
         s16x2 t2n = t2;
         if (r + 1 < R) t2n = __builtin_elementwise_add_sat(S2(H[r]), S2(sv[r + 1]));
-        const uint32_t hn =
-            W(__builtin_elementwise_max(__builtin_elementwise_max(t2, S2(E[r])), S2(f)));
+        const uint32_t hn = max3h(W(t2), E[r], f);
         H[r] = hn;
-        const u16x2 ho = __builtin_elementwise_sub_sat(U2(hn), U2(gap_o));
-        E[r] = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))));
-        f = W(__builtin_elementwise_max(ho, __builtin_elementwise_sub_sat(U2(f), U2(gap_x))));
+        const uint32_t ho = W(__builtin_elementwise_sub_sat(U2(hn), U2(gap_o)));
+        E[r] = max3h(ho, W(__builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))), kFloor2);
+        f = max3h(ho, W(__builtin_elementwise_sub_sat(U2(f), U2(gap_x))), kFloor2);
         if constexpr (PAIRKEY) {
           const uint32_t jr = (r & 1) ? jr_odd : jr_even;
           const uint32_t kl = (hn << 16) | jr;
@@ -819,7 +838,7 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
   int bh_lo = 0, bi_lo = 0, bj_lo = 0, bh_hi = 0, bi_hi = 0, bj_hi = 0;
 #pragma unroll
   for (int q = 0; q < KN; ++q) {
-    const int hl = (int)(KL[q] >> 16);
+    const int hl = (int)(KL[q] >> 16) - kBias16;
     if (hl > bh_lo) {
       bh_lo = hl;
       if constexpr (PAIRKEY) {
@@ -830,7 +849,7 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
         bj_lo = 0xFFFF - (int)(KL[q] & 0xFFFFu);
       }
     }
-    const int hh = (int)(KH[q] >> 16);
+    const int hh = (int)(KH[q] >> 16) - kBias16;
     if (hh > bh_hi) {
       bh_hi = hh;
       if constexpr (PAIRKEY) {

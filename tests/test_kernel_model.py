@@ -3,8 +3,12 @@
 It replays, for one comparison, what k_fast does lane by lane: super-strips of
 L*R rows, lane k computing column j = t - k at step t, H/F/code handed to lane
 k+1 each step, the row above each super-strip read from / written to a
-boundary row, E and F clamped at 0 with unsigned saturating subtraction, the
-substitution add saturating at 32767, pad code scores of -32768, per-row keys
+boundary row, every value held plus 1,024 (so that H, E and F are positive
+normal fp16 bit patterns and H = max3(diag + s, E, F), E = max3(H - o, E - x,
+1024) are fp16 maxima of three, NaN-propagating, on those patterns), E and F
+clamped at 0 with unsigned saturating subtraction, the substitution add
+saturating at 32767 (a biased H at 0x7C00 or above flags the comparison for the
+int32 re-run), pad code scores of -32768, per-row keys
 H << 16 | (0xFFFF - j) -- or, for pipelines of up to 16 lanes, one key per
 row pair, H << 16 | (even row) << 15 | (0x7FFF - j) -- rows merged with a
 strict > and lanes by (max H, min row).  With tagkey, a super-strip whose H
@@ -20,7 +24,17 @@ import pytest
 
 import oracle as O
 
-SAT = 32767
+BIAS = 1024
+SAT = 0x7C00 - BIAS  # the re-run threshold of a comparison's maximum
+
+
+def max3(*vals):
+    """v_pk_maximum3_f16 on one half: IEEE maximum of the 16-bit patterns read
+    as fp16 (a NaN operand gives the quiet NaN 0x7E00)."""
+    f = np.array([v & 0xFFFF for v in vals], np.uint16).view(np.float16)
+    if np.isnan(f).any():
+        return 0x7E00
+    return int(np.array([f.max()], np.float16).view(np.uint16)[0])
 
 
 def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
@@ -32,8 +46,8 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
     table[:dim, :dim] = mat
     mpad = m if mpad is None else mpad
     nss = -(-n // (L * R))
-    bnd_h = np.zeros(mpad, np.int64)
-    bnd_f = np.zeros(mpad, np.int64)
+    bnd_h = np.full(mpad, BIAS, np.int64)
+    bnd_f = np.full(mpad, BIAS, np.int64)
     best = [(0, 0, 0)] * L  # per lane (h, i, j)
     smax = max(0, int(np.max(mat)))
     tags_used = 0
@@ -44,17 +58,17 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
     for ss in range(nss):
         rows = [[(ss * L + k) * R + r for r in range(R)] for k in range(L)]
         bcode = [[b[i] if i < n else pad for i in rows[k]] for k in range(L)]
-        H = np.zeros((L, R), np.int64)
-        E = np.zeros((L, R), np.int64)
+        H = np.full((L, R), BIAS, np.int64)
+        E = np.full((L, R), BIAS, np.int64)
         key = np.zeros((L, R), np.int64)
         pkey = np.zeros((L, (R + 1) // 2), np.int64)
         use_tag = tagkey and R <= 16 and max(bh for bh, _, _ in best) + smax * R * L <= 4095
         tags_used += use_tag
         tkey = [0] * L
-        out_h = [0] * L
-        out_f = [0] * L
+        out_h = [BIAS] * L
+        out_f = [BIAS] * L
         col = [pad] * L
-        hdiag = [0] * L
+        hdiag = [BIAS] * L
         new_bh = bnd_h.copy()
         new_bf = bnd_f.copy()
         T = mpad + L - 1
@@ -63,8 +77,8 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
             prev_h, prev_f, prev_c = out_h[:], out_f[:], col[:]
             for k in range(L):
                 if k == 0:
-                    rh = bnd_h[t] if (ss > 0 and t < mpad) else 0
-                    rf = bnd_f[t] if (ss > 0 and t < mpad) else 0
+                    rh = bnd_h[t] if (ss > 0 and t < mpad) else BIAS
+                    rf = bnd_f[t] if (ss > 0 and t < mpad) else BIAS
                     col[k] = a[t] if t < m else pad
                 else:
                     rh, rf = prev_h[k - 1], prev_f[k - 1]
@@ -78,17 +92,18 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
                 k16 = 0
                 for r in range(R):
                     s = table[col[k], bcode[k][r]]
-                    t2 = min(d + s, SAT)
-                    hn = max(t2, E[k, r], f)
+                    # saturating int16 add of the biased diagonal, as a bit pattern
+                    t2 = (min(max(int(np.int16(d)) + s, -32768), 32767)) & 0xFFFF
+                    hn = max3(t2, E[k, r], f)
                     ho = subsat(hn, o)
-                    E[k, r] = max(ho, subsat(E[k, r], x))
-                    f = max(ho, subsat(f, x))
+                    E[k, r] = max3(ho, subsat(E[k, r], x), BIAS)
+                    f = max3(ho, subsat(f, x), BIAS)
                     key[k, r] = max(key[k, r], (hn << 16) | jinv)
                     pkey[k, r // 2] = max(pkey[k, r // 2],
                                           (hn << 16) | ((r % 2 == 0) << 15) | j15)
                     if use_tag:
-                        assert hn <= 4095, (hn, ss)  # the super-strip's bound
-                        k16 = max(k16, hn * 16 + 15 - r)
+                        assert hn - BIAS <= 4095, (hn, ss)  # the super-strip's bound
+                        k16 = max(k16, (hn * 16 + 15 - r - 16 * BIAS) & 0xFFFF)
                     d = H[k, r]
                     H[k, r] = hn
                 if use_tag:
@@ -108,13 +123,13 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
                 continue
             if pairkey:
                 for q in range((R + 1) // 2):
-                    h = int(pkey[k, q] >> 16)
+                    h = int(pkey[k, q] >> 16) - BIAS
                     if h > best[k][0]:
                         r = 2 * q + (0 if pkey[k, q] & 0x8000 else 1)
                         best[k] = (h, rows[k][r], 0x7FFF - int(pkey[k, q] & 0x7FFF))
                 continue
             for r in range(R):
-                h = int(key[k, r] >> 16)
+                h = int(key[k, r] >> 16) - BIAS
                 if h > best[k][0]:
                     best[k] = (h, rows[k][r], 0xFFFF - int(key[k, r] & 0xFFFF))
     h, i, j = 0, 0, 0
@@ -163,13 +178,18 @@ def test_model_matches_oracle_protein():
                              O.unpack_range(br[0])[1] if h else 0)
 
 
-def test_model_flags_saturation():
-    # an exact score >= 32767 must surface as 32767 (the int32 re-run flag)
+@pytest.mark.parametrize("n", [242, 260, 300])
+def test_model_flags_saturation(n):
+    # an exact score at or above SAT must surface at or above SAT (the int32
+    # re-run flag); 241 * 127 = 30607 stays below it and is exact
     mat = np.full((25, 25), -1, np.int64)
     np.fill_diagonal(mat, 127)
-    a = np.zeros(300, np.int64)  # 300 * 127 = 38100 > 32767
+    a = np.zeros(n, np.int64)  # n * 127 >= 30734 > SAT
     h, _, _ = model(a, a, mat, -10, -1, L=2, R=8)
-    assert h == SAT
+    assert h >= SAT
+    b = np.zeros(241, np.int64)
+    h, i, j = model(b, b, mat, -10, -1, L=2, R=8)
+    assert (h, i, j) == (241 * 127, 240, 240)
 
 
 @pytest.mark.parametrize("L,R", [(1, 16), (2, 8), (4, 4), (3, 12)])
