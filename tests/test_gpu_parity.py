@@ -429,6 +429,40 @@ def test_int16_saturation_rerun(gpu):
     assert_same(got, oracle_batch(batch, False), "saturation")
 
 
+def test_saturation_threshold(gpu):
+    # the packed kernel holds H + 1024 below 0x7C00: W^2792 (30,712) stays
+    # exact in the packed kernel, W^2793 (30,723) is re-run in int32
+    note("saturation threshold: W^2792 packed, W^2793 re-run")
+    batch = W.make_batch([(np.full(2792, 17, np.uint8), np.full(2792, 17, np.uint8)),
+                          (np.full(2793, 17, np.uint8), np.full(2793, 17, np.uint8))],
+                         "thr", W.PROTEIN_SW)
+    eng = engine_for(batch["sw"], False)
+    got = device_run(eng, batch)
+    assert [int(x) for x in got[0]] == [11 * 2792, 11 * 2793]
+    assert eng.stats()["rerun_pairs"] == 1
+    assert_same(got, oracle_batch(batch, False), "threshold")
+
+
+@pytest.mark.parametrize("lanes", [16, 64])
+def test_high_scores_near_tag_limit(gpu, lanes):
+    # match 16 over |A| <= 250: maxima up to 4,000, just below the largest H a
+    # row-tag key holds, on 16- and 64-lane pipelines (whose super-strips take
+    # 32-bit keys: 16 x their rows exceeds it); with starts
+    note(f"high scores near the row-tag limit, lanes={lanes}")
+    rng = np.random.default_rng(31 + lanes)
+    pairs = []
+    for _ in range(120):
+        a = W._dna_draw(rng, int(rng.integers(200, 251)))
+        pairs.append((a, W._mutate(rng, a, 0.02, 0.01, W._dna_draw, W._dna_subst)))
+        pairs.append((a, np.concatenate([W._dna_draw(rng, int(rng.integers(0, 300))), a])))
+    batch = W.make_batch(pairs, "colbound", dict(W.DNA_SW, matchValue=16, mismatchValue=-3))
+    eng = engine_for(batch["sw"], False)
+    eng.set_lanes_per_pair(lanes)
+    got = device_run(eng, batch)
+    assert_same(got, oracle_batch(batch, False), f"high scores, lanes={lanes}")
+    assert int(got[0].max()) >= 16 * 240
+
+
 @pytest.mark.parametrize("forward_only", [True, False])
 def test_int32_rerun_long_pairs(gpu, forward_only):
     """Long saturating pairs (2-4 kb, every score above 32,767) through the
