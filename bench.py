@@ -67,8 +67,9 @@ def parse():
     ap.add_argument("--rounds", type=int, default=0,
                     help="groups per resident wave for the bucketing (0 = engine default)")
     ap.add_argument("--cap-scale", type=float, default=0.0,
-                    help="step cap scale of the protein batch's bucketing (0 = engine default or "
-                         "the autotuner's choice; set: no autotuning of that batch)")
+                    help="step cap scale of the bucketing of the --config workload (0 = engine "
+                         "default or the autotuner's choice; set: no autotuning of that workload; "
+                         "the sub-records keep their autotuning)")
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="time a small grid of bucketing shapes (rounds x step cap scale) on "
                          "each batch before its timed region and keep the fastest (off with "
@@ -358,7 +359,10 @@ def bench_skewed(args, peak_cal):
     log(f"skewed: generated {len(b['a_len'])} pairs, {b['cells']:.3e} cells in "
         f"{time.time() - t:.1f}s")
     db = DeviceBatch(b)
-    tuned = db.tune("dna", reps=2) if args.autotune == "on" else None
+    fixed = args.config == "skewed" and args.cap_scale
+    if fixed:
+        db.eng.set_step_cap_scale(args.cap_scale)
+    tuned = db.tune("dna", reps=2) if args.autotune == "on" and not fixed else None
     ms, kms, st = db.timed(args.sub_steps, 1)
     got = db.results()
     pms, pst = db.plain(I.BATCH_PLAIN_TILE, 1)
@@ -385,6 +389,7 @@ def bench_skewed(args, peak_cal):
            "roofline": roofline(cells, kms, "k_fast<18, false>", peak_cal),
            "autotune": ({"rounds": tuned["rounds"], "step_cap_scale": tuned["scale"]}
                         if tuned else None),
+           "step_cap_scale": args.cap_scale if fixed else None,
            "stats": {k: st[k] for k in ("groups", "padded_cells", "band_groups")}}
     log(f"skewed: {rec['ms_per_step']} ms/step, {rec['gcups']} GCUPS, plain "
         f"{rec['plain']['ms_per_step']} ms ({rec['bucketed_vs_plain']}x), parity {rec['parity']}")
@@ -420,7 +425,10 @@ def bench_search(args, peak_cal, cpu_seconds=0.0):
                           d["db_len"], k, scores, ti, ts, stream=sptr)
 
     tuned = None
-    if args.autotune == "on" and not args.rounds:
+    fixed = args.config == "search" and args.cap_scale
+    if fixed:
+        eng.set_step_cap_scale(args.cap_scale)
+    if args.autotune == "on" and not args.rounds and not fixed:
         from ipuma_amd import autotune as AT
         dbs = b["db_len"][:4096]  # pairs of every query with a database sample
         key = AT.signature(np.repeat(b["q_len"], len(dbs)), np.tile(dbs, nq), "search-blosum62")
@@ -489,6 +497,7 @@ def bench_search(args, peak_cal, cpu_seconds=0.0):
            "roofline": roofline(cells, fast_avg, "k_fast<26, false>", peak_cal),
            "autotune": ({"rounds": tuned["rounds"], "step_cap_scale": tuned["scale"]}
                         if tuned else None),
+           "step_cap_scale": args.cap_scale if fixed else None,
            "cpu_baseline": cpu,
            "stats": {kk: st[kk] for kk in ("groups", "padded_cells", "band_groups")}}
     log(f"search: {rec['ms_per_step']} ms/step, {rec['gcups']} GCUPS, parity {rec['parity']}")

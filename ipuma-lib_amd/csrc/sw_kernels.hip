@@ -445,8 +445,11 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
         }
         out_h = H[R - 1];
         out_f = f;
-        if (last && store_bnd && j >= 0 && j < mpad)
-          *(uint2 *)((char *)wscr + (uint32_t)((j * NG + g) * 8)) = make_uint2(out_h, out_f);
+        // the last lane of every pipeline is at column t - (L - 1): a uniform
+        // (scalar) range test, then the lane mask
+        const int jl = t - (L - 1);
+        if (store_bnd && (uint32_t)jl < (uint32_t)mpad && last)
+          *(uint2 *)((char *)wscr + (uint32_t)((jl * NG + g) * 8)) = make_uint2(out_h, out_f);
       }
     }
     // rows (or row pairs) in order, strict >: the first maximum in B order
@@ -821,8 +824,9 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
       }
       out_h = H[R - 1];
       out_f = f;
-      if (last && store_bnd && j >= 0 && j < mpad)
-        *(uint2 *)(bscr + (size_t)j * 2) = make_uint2(out_h, out_f);
+      // lane 63 is at column t - 63: a uniform range test, then the lane mask
+      if (store_bnd && (uint32_t)(t - 63) < (uint32_t)mpad && last)
+        *(uint2 *)(bscr + (size_t)(t - 63) * 2) = make_uint2(out_h, out_f);
     }
     // publish the boundary columns written so far (lane 63 wrote j = t - 63)
     if (store_bnd && (t0 & 15) == 12) {
