@@ -1120,6 +1120,7 @@ template __global__ void k_fast<26, true>(FastArgs);
 // packed kernel's domain (and the int32 precision policy).
 // ---------------------------------------------------------------------------
 constexpr int kW32Rows = 16;
+static_assert(kW32Rows <= 16, "k_wave32 row keys hold the row in 4 bits");
 
 __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
   constexpr int R = kW32Rows;
@@ -1132,6 +1133,7 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
   int32_t *const bnd = G.work + (int64_t)blockIdx.x * G.work_stride;
   const int total = G.list ? min(*G.count, G.n) : G.n;
   const int gI = G.gap_init, gE = G.gap_extend, gIE = gI + gE;
+  const bool pen = gIE <= 0 && gE <= 0;
   const bool first = lane == 0, last = lane == 63;
   for (int idx = blockIdx.x; idx < total; idx += gridDim.x) {
     const int p = G.list ? G.list[idx] : idx;
@@ -1167,87 +1169,138 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
       }
       const int i0 = (ss * 64 + lane) * R;
       const bool load_bnd = ss > 0, store_bnd = ss + 1 < nss;
-      int boff[R], H[R], E[R], rmax[R], rcol[R];
+      // Gaps that cost (gI + gE <= 0, gE <= 0) bound every H by |A| * 127 <
+      // 2^23: each row's H then becomes a key H * 16 + (15 - r), and a max over
+      // the rows gives the step's (max H, first row).  Other gap settings keep
+      // a maximum and its column per row.
+      auto run = [&](auto pen_c) {
+        constexpr bool PEN = decltype(pen_c)::value;
+        int boff[R], H[R], E[R], rmax[PEN ? 1 : R], rcol[PEN ? 1 : R], toff[PEN ? R : 1];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        boff[r] = i0 + r < n ? min((int)b[i0 + r], kp - 1) : 0;
-        H[r] = 0;   // H(i, -1)
-        E[r] = gI;  // E(i, -1)
-        // a row past |B| never takes a maximum (nothing exceeds INT_MAX): the
-        // step needs no row check, the merge below skips such rows
-        rmax[r] = i0 + r < n ? 0 : INT32_MAX;
-        rcol[r] = 0;
-      }
-      int out_h = 0, out_f = 0, col = 0;
-      int hdiag = 0;  // H(i0 - 1, j - 1)
-      // Lane 0's inputs at step t (column t: its A code and the boundary row's
-      // H and F) come from blocks of 64 columns that every lane loads one
-      // column of, a block ahead; the wave reads lane t % 64's values with
-      // v_readlane (a uniform lane index: no LDS round trip).  Lane 63
-      // overwrites column c of the boundary at step c + 63, after the load of
-      // c's block (at step c - 64 or earlier).
-      int a_cur, h_cur = 0, f_cur = 0, a_nxt, h_nxt = 0, f_nxt = 0;
-      auto load_block = [&](int c0, int &av, int &hv, int &fv) {
-        const int c = c0 + lane;
-        av = c < m ? min((int)a[c], kp - 1) * kp : 0;
-        if (load_bnd && c < m) {
-          hv = bnd[2 * c];
-          fv = bnd[2 * c + 1];
+        for (int r = 0; r < R; ++r) {
+          boff[r] = i0 + r < n ? min((int)b[i0 + r], kp - 1) : 0;
+          H[r] = 0;  // H(i, -1)
+          if constexpr (PEN) {
+            // E of the row at the lane's next column: E(i, 0) from H(i, -1) = 0
+            // and E(i, -1) = gI; a row past |B| keys below every real row
+            E[r] = max(gIE, gI + gE);
+            toff[r] = i0 + r < n ? 15 - r : -(1 << 30);
+          } else {
+            E[r] = gI;  // E(i, -1)
+            // a row past |B| never takes a maximum (nothing exceeds INT_MAX):
+            // the step needs no row check, the merge below skips such rows
+            rmax[r] = i0 + r < n ? 0 : INT32_MAX;
+            rcol[r] = 0;
+          }
+        }
+        int bkey = INT32_MIN, bcol = 0;  // the lane's best key in this super-strip, its column
+        int out_h = 0, out_f = 0, col = 0;
+        int hdiag = 0;  // H(i0 - 1, j - 1)
+        // Lane 0's inputs at step t (column t: its A code and the boundary row's
+        // H and F) come from blocks of 64 columns that every lane loads one
+        // column of, a block ahead; the wave reads lane t % 64's values with
+        // v_readlane (a uniform lane index: no LDS round trip).  Lane 63
+        // overwrites column c of the boundary at step c + 63, after the load of
+        // c's block (at step c - 64 or earlier).
+        int a_cur, h_cur = 0, f_cur = 0, a_nxt, h_nxt = 0, f_nxt = 0;
+        auto load_block = [&](int c0, int &av, int &hv, int &fv) {
+          const int c = c0 + lane;
+          av = c < m ? min((int)a[c], kp - 1) * kp : 0;
+          if (load_bnd && c < m) {
+            hv = bnd[2 * c];
+            fv = bnd[2 * c + 1];
+          }
+        };
+        load_block(0, a_cur, h_cur, f_cur);
+        load_block(64, a_nxt, h_nxt, f_nxt);
+        const int T = m + 63;
+        for (int t = 0; t < T; ++t) {
+          const int j = t - lane;
+          if (t > 0 && (t & 63) == 0) {
+            a_cur = a_nxt;
+            h_cur = h_nxt;
+            f_cur = f_nxt;
+            load_block(t + 64, a_nxt, h_nxt, f_nxt);
+          }
+          const int sl = t & 63;
+          const int ac = __builtin_amdgcn_readlane(a_cur, sl);
+          const int bh = load_bnd ? __builtin_amdgcn_readlane(h_cur, sl) : 0;
+          const int bf = load_bnd ? __builtin_amdgcn_readlane(f_cur, sl) : 0;
+          const int up_h = (int)from_prev<kModeWave>((uint32_t)bh, (uint32_t)out_h, first);
+          const int up_f = (int)from_prev<kModeWave>((uint32_t)bf, (uint32_t)out_f, first);
+          col = (int)from_prev<kModeWave>((uint32_t)ac, (uint32_t)col, first);
+          if (j >= 0 && j < m) {
+            int hd = hdiag, hup = up_h, fup = up_f;
+            if constexpr (PEN) {
+              // H(i - 1, j) + gI + gE carried down the rows with F
+              int hoo = up_h + gIE, kev = 0, tk = INT32_MIN;
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const int sc = tab[col + boff[r]];
+                const int e = E[r];  // E(i, j)
+                const int f = max(hoo, fup + gE);
+                const int h = max(max(hd + sc, max(e, f)), 0);
+                hd = H[r];
+                H[r] = h;
+                const int ho = h + gIE;
+                E[r] = max(ho, e + gE);  // E(i, j + 1)
+                hoo = ho;
+                fup = f;
+                const int kr = (h << 4) + toff[r];
+                if (r & 1) tk = max(tk, max(kev, kr));
+                else kev = kr;
+              }
+              hup = H[R - 1];
+              if (tk > bkey) {  // strict: the first column of the lane's best key
+                bkey = tk;
+                bcol = j;
+              }
+            } else {
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const int sc = tab[col + boff[r]];
+                const int e = max(H[r] + gIE, E[r] + gE);
+                const int f = max(hup + gIE, fup + gE);
+                const int h = max(max(hd + sc, e), max(f, 0));
+                hd = H[r];
+                H[r] = h;
+                E[r] = e;
+                hup = h;
+                fup = f;
+                if (h > rmax[r]) {
+                  rmax[r] = h;
+                  rcol[r] = j;
+                }
+              }
+            }
+            out_h = hup;
+            out_f = fup;
+            if (last && store_bnd) {
+              bnd[2 * j] = hup;
+              bnd[2 * j + 1] = fup;
+            }
+            hdiag = up_h;
+          }
+        }
+        // rows in order, strict >: the lane's first maximum in B order so far
+        if constexpr (PEN) {
+          if (bkey >= 0 && (bkey >> 4) > best) {
+            best = bkey >> 4;
+            bi = i0 + 15 - (bkey & 15);
+            bj = bcol;
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (i0 + r < n && rmax[r] > best) {
+              best = rmax[r];
+              bi = i0 + r;
+              bj = rcol[r];
+            }
         }
       };
-      load_block(0, a_cur, h_cur, f_cur);
-      load_block(64, a_nxt, h_nxt, f_nxt);
-      const int T = m + 63;
-      for (int t = 0; t < T; ++t) {
-        const int j = t - lane;
-        if (t > 0 && (t & 63) == 0) {
-          a_cur = a_nxt;
-          h_cur = h_nxt;
-          f_cur = f_nxt;
-          load_block(t + 64, a_nxt, h_nxt, f_nxt);
-        }
-        const int sl = t & 63;
-        const int ac = __builtin_amdgcn_readlane(a_cur, sl);
-        const int bh = load_bnd ? __builtin_amdgcn_readlane(h_cur, sl) : 0;
-        const int bf = load_bnd ? __builtin_amdgcn_readlane(f_cur, sl) : 0;
-        const int up_h = (int)from_prev<kModeWave>((uint32_t)bh, (uint32_t)out_h, first);
-        const int up_f = (int)from_prev<kModeWave>((uint32_t)bf, (uint32_t)out_f, first);
-        col = (int)from_prev<kModeWave>((uint32_t)ac, (uint32_t)col, first);
-        if (j >= 0 && j < m) {
-          int hd = hdiag, hup = up_h, fup = up_f;
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const int sc = tab[col + boff[r]];
-            const int e = max(H[r] + gIE, E[r] + gE);
-            const int f = max(hup + gIE, fup + gE);
-            const int h = max(max(hd + sc, e), max(f, 0));
-            hd = H[r];
-            H[r] = h;
-            E[r] = e;
-            hup = h;
-            fup = f;
-            if (h > rmax[r]) {
-              rmax[r] = h;
-              rcol[r] = j;
-            }
-          }
-          out_h = hup;
-          out_f = fup;
-          if (last && store_bnd) {
-            bnd[2 * j] = hup;
-            bnd[2 * j + 1] = fup;
-          }
-          hdiag = up_h;
-        }
-      }
-      // rows in order, strict >: the lane's first maximum in B order so far
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        if (i0 + r < n && rmax[r] > best) {
-          best = rmax[r];
-          bi = i0 + r;
-          bj = rcol[r];
-        }
+      if (pen) run(std::true_type{});
+      else run(std::false_type{});
     }
     // lanes hold disjoint rows: (max H, min row)
     for (int off = 1; off < 64; off <<= 1) {
