@@ -506,14 +506,23 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   // pipeline's best only together with the other lanes: the ballot covers
   // every lane) plus A.smax times its rows.  Row-tag keys hold when that stays
   // within kTagKeyMaxH for every pair of the wave.
+  //
+  // A group of one super-strip (nothing above it, nothing after it) takes
+  // row-tag keys even when the bound does not hold, and checks afterwards: an
+  // H above kTagKeyMaxH is reached from 0 by steps of at most A.smax up the
+  // diagonals (gaps only lower H), so some cell then holds an H within A.smax
+  // of kTagKeyMaxH, whose key is exact.  When no lane's maximum came that close,
+  // every key was exact; otherwise the super-strip runs again with 32-bit keys.
   auto run_strip = [&](auto rows, const int ss) {
     constexpr int R = decltype(rows)::value;
     if constexpr (PAIRKEY) {
       const int lim = kTagKeyMaxH - A.smax * (R << lg);
-      if (__ballot(max(bh_lo, bh_hi) > lim) == 0) {
+      const bool bound = __ballot(max(bh_lo, bh_hi) > lim) == 0;
+      if (bound || nss == 1) {
         if (MODE == kModeRow && shared) strip(rows, std::true_type{}, std::true_type{}, ss);
         else strip(rows, std::true_type{}, std::false_type{}, ss);
-        return;
+        if (bound || __ballot(max(bh_lo, bh_hi) > kTagKeyMaxH - A.smax) == 0) return;
+        bh_lo = bi_lo = bj_lo = bh_hi = bi_hi = bj_hi = 0;
       }
     }
     strip(rows, std::false_type{}, std::false_type{}, ss);

@@ -463,6 +463,26 @@ def test_high_scores_near_tag_limit(gpu, lanes):
     assert int(got[0].max()) >= 16 * 240
 
 
+def test_single_strip_tag_fallback(gpu):
+    # groups of one 64-lane super-strip take row-tag keys without the bound
+    # and run again with 32-bit keys when a maximum comes near 4,095: match 20
+    # over 120-300 matching residues scores 2,400-6,000, on both sides
+    note("single super-strips: row-tag keys kept below 4,075, re-run above")
+    rng = np.random.default_rng(41)
+    pairs = []
+    for _ in range(160):
+        n = int(rng.integers(120, 301))
+        a = W._dna_draw(rng, n + 60)
+        pairs.append((a, a[30: 30 + n]))
+        pairs.append((a, W._mutate(rng, a[: n], 0.05, 0.01, W._dna_draw, W._dna_subst)))
+    batch = W.make_batch(pairs, "tagfall", dict(W.DNA_SW, matchValue=20, mismatchValue=-3))
+    eng = engine_for(batch["sw"], False)
+    eng.set_lanes_per_pair(64)
+    got = device_run(eng, batch)
+    assert_same(got, oracle_batch(batch, False), "single strip tag fallback")
+    assert int(got[0].min()) < 4000 < 4200 < int(got[0].max())
+
+
 @pytest.mark.parametrize("forward_only", [True, False])
 def test_int32_rerun_long_pairs(gpu, forward_only):
     """Long saturating pairs (2-4 kb, every score above 32,767) through the

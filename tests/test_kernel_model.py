@@ -37,7 +37,7 @@ def max3(*vals):
     return int(np.array([f.max()], np.float16).view(np.uint16)[0])
 
 
-def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
+def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False, force_tag=False):
     m, n = len(a), len(b)
     o, x = -(gI + gE), -gE
     dim = mat.shape[0]
@@ -62,7 +62,8 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
         E = np.full((L, R), BIAS, np.int64)
         key = np.zeros((L, R), np.int64)
         pkey = np.zeros((L, (R + 1) // 2), np.int64)
-        use_tag = tagkey and R <= 16 and max(bh for bh, _, _ in best) + smax * R * L <= 4095
+        use_tag = force_tag or (tagkey and R <= 16 and
+                                max(bh for bh, _, _ in best) + smax * R * L <= 4095)
         tags_used += use_tag
         tkey = [0] * L
         out_h = [BIAS] * L
@@ -102,7 +103,7 @@ def model(a, b, mat, gI, gE, L, R, mpad=None, pairkey=False, tagkey=False):
                     pkey[k, r // 2] = max(pkey[k, r // 2],
                                           (hn << 16) | ((r % 2 == 0) << 15) | j15)
                     if use_tag:
-                        assert hn - BIAS <= 4095, (hn, ss)  # the super-strip's bound
+                        assert force_tag or hn - BIAS <= 4095, (hn, ss)  # the bound
                         k16 = max(k16, (hn * 16 + 15 - r - 16 * BIAS) & 0xFFFF)
                     d = H[k, r]
                     H[k, r] = hn
@@ -222,3 +223,44 @@ def test_model_row_tag_keys(L, R):
         s, ar, br = O.sw_batch_codes(codes, [0], [len(a)], [len(a)], [len(b)], mat, gI, gE, True)
         assert (h, j, i) == (int(s[0]), O.unpack_range(ar[0])[1] if h else 0,
                              O.unpack_range(br[0])[1] if h else 0), (trial, match)
+
+
+def model_single_strip(a, b, mat, gI, gE, L, R, mpad):
+    """The kernel's rule for a group of one super-strip: row-tag keys when the
+    bound holds; otherwise row-tag keys anyway, kept unless the maximum came
+    within smax of 4,095, else a second run with 32-bit keys.  Returns the
+    result and the forced run's maximum (None when the bound held)."""
+    smax = max(0, int(np.max(mat)))
+    res = model(a, b, mat, gI, gE, L, R, mpad, pairkey=True, tagkey=True)
+    if model.tags_used:
+        return res, None
+    forced = model(a, b, mat, gI, gE, L, R, mpad, pairkey=True, force_tag=True)
+    if forced[0] <= 4095 - smax:
+        return forced, forced[0]
+    return model(a, b, mat, gI, gE, L, R, mpad, pairkey=True), forced[0]
+
+
+def test_model_single_strip_optimistic_tags():
+    """Single super-strips (|B| <= L * R) whose rows' bound fails: DNA at match
+    20 on 16 lanes x 16 rows.  Below the limit the forced row-tag keys are
+    exact; above it (scores up to 20 x 256) the forced run's maximum comes
+    within smax of 4,095, so the kernel falls back to 32-bit keys.  Either way
+    the result equals the oracle's."""
+    rng = np.random.default_rng(9)
+    mat = O.select_matrix(O.SIM_NA, 20, -3, -1)
+    fell_back = kept = 0
+    for trial in range(10):
+        n = int(rng.integers(100, 257))
+        a = rng.choice([1, 2, 3, 4], n + 40).astype(np.int64)
+        b = a[20: 20 + n] if trial % 2 == 0 else np.concatenate([rng.choice([1, 2, 3, 4], 7), a[: n - 40]])
+        (h, i, j), fmax = model_single_strip(a, b, mat, -2, -1, 16, 16, len(a) + 2)
+        codes = np.concatenate([a, b]).astype(np.uint8)
+        s, ar, br = O.sw_batch_codes(codes, [0], [len(a)], [len(a)], [len(b)], mat, -2, -1, True)
+        assert (h, j, i) == (int(s[0]), O.unpack_range(ar[0])[1], O.unpack_range(br[0])[1])
+        assert fmax is not None  # 20 x 256 > 4,095: the bound never holds here
+        if int(s[0]) > 4095:
+            assert fmax > 4095 - 20  # the overflow is always detected
+            fell_back += 1
+        elif fmax <= 4095 - 20:
+            kept += 1
+    assert fell_back >= 2 and kept >= 2, (fell_back, kept)
