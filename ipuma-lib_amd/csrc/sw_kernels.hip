@@ -507,22 +507,26 @@ __device__ __forceinline__ void run_group(const FastArgs &A, const uint32_t *__r
   // every lane) plus A.smax times its rows.  Row-tag keys hold when that stays
   // within kTagKeyMaxH for every pair of the wave.
   //
-  // A group of one super-strip (nothing above it, nothing after it) takes
-  // row-tag keys even when the bound does not hold, and checks afterwards: an
-  // H above kTagKeyMaxH is reached from 0 by steps of at most A.smax up the
-  // diagonals (gaps only lower H), so some cell then holds an H within A.smax
-  // of kTagKeyMaxH, whose key is exact.  When no lane's maximum came that close,
-  // every key was exact; otherwise the super-strip runs again with 32-bit keys.
+  // The last super-strip of a group (nothing after it reads its boundary row,
+  // so it can run again) takes row-tag keys even when the bound does not hold,
+  // as long as the best H above it stays within kTagKeyMaxH - A.smax, and
+  // checks afterwards: an H above kTagKeyMaxH is reached from below that by
+  // steps of at most A.smax up the diagonals (gaps only lower H), so some cell
+  // of the super-strip then holds an H within A.smax of kTagKeyMaxH, whose key
+  // is exact.  When no lane's maximum came that close, every key was exact;
+  // otherwise the super-strip runs again with 32-bit keys.
   auto run_strip = [&](auto rows, const int ss) {
     constexpr int R = decltype(rows)::value;
     if constexpr (PAIRKEY) {
       const int lim = kTagKeyMaxH - A.smax * (R << lg);
+      const int near = kTagKeyMaxH - A.smax;
       const bool bound = __ballot(max(bh_lo, bh_hi) > lim) == 0;
-      if (bound || nss == 1) {
+      if (bound || (ss + 1 == nss && __ballot(max(bh_lo, bh_hi) > near) == 0)) {
+        const int s_hl = bh_lo, s_il = bi_lo, s_jl = bj_lo, s_hh = bh_hi, s_ih = bi_hi, s_jh = bj_hi;
         if (MODE == kModeRow && shared) strip(rows, std::true_type{}, std::true_type{}, ss);
         else strip(rows, std::true_type{}, std::false_type{}, ss);
-        if (bound || __ballot(max(bh_lo, bh_hi) > kTagKeyMaxH - A.smax) == 0) return;
-        bh_lo = bi_lo = bj_lo = bh_hi = bi_hi = bj_hi = 0;
+        if (bound || __ballot(max(bh_lo, bh_hi) > near) == 0) return;
+        bh_lo = s_hl, bi_lo = s_il, bj_lo = s_jl, bh_hi = s_hh, bi_hi = s_ih, bj_hi = s_jh;
       }
     }
     strip(rows, std::false_type{}, std::false_type{}, ss);

@@ -351,6 +351,38 @@ def test_band_groups_emulated(emu_lib):
         eng.close()
 
 
+def test_last_strip_optimistic_tags_emulated(emu_lib):
+    """The last super-strip of a group takes row-tag keys without the a-priori
+    bound and runs again with 32-bit keys when a maximum comes within smax of
+    4,095: 16-lane pipelines (super-strips of 256 rows), B = 260 random rows +
+    a 190- or 240-residue copy of A, match 20 (scores near 3,800 keep the keys,
+    near 4,800 re-run), one pair on a single super-strip too; with starts,
+    against the oracle."""
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(77)
+    A, B = [], []
+    for k in range(6):
+        a = "".join(rng.choice(list("ACGT"), 260))
+        n = 190 if k % 2 == 0 else 240
+        B.append("".join(rng.choice(list("ACGT"), 260)) + a[10: 10 + n])
+        A.append(a)
+    A.append(A[0])
+    B.append(A[0][:230])  # one super-strip, 4,600
+    # mismatches and gaps cost enough that the random rows add nothing
+    exp = O.sw_batch(A, B, match=20, mismatch=-20, ambiguity=-20, gap_init=-20, gap_extend=-5,
+                     forward_only=False)
+    assert max(exp[0]) > 4095 > 4075 > min(exp[0])
+    eng = I.SWAlgorithm(I.SWConfig(gapInit=-20, gapExtend=-5, matchValue=20, mismatchValue=-20,
+                                   ambiguityValue=-20),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    eng.set_lanes_per_pair(16)
+    got = eng.compare(A, B)
+    for g, e, nm in zip(got, exp, ("scores", "a_range", "b_range")):
+        assert np.array_equal(np.asarray(g), np.asarray(e)), nm
+    eng.close()
+
+
 def _tie_pair(rng, n):
     """A = X + Y, B = Y + X with |X| = |Y| = n: two equal best alignments, the
     one on the earlier B row ending at the later A column."""

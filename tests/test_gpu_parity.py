@@ -483,6 +483,30 @@ def test_single_strip_tag_fallback(gpu):
     assert int(got[0].min()) < 4000 < 4200 < int(got[0].max())
 
 
+@pytest.mark.parametrize("lanes", [8, 16])
+def test_last_strip_tag_fallback(gpu, lanes):
+    # the last super-strip of a group takes row-tag keys without the bound and
+    # runs again with 32-bit keys when a maximum comes near 4,095: B = random
+    # rows (one or more super-strips) + a 150-260-residue copy of A, match 20,
+    # with starts (scores about 20 x the copy: 3,000-5,200)
+    note(f"last super-strips: row-tag keys kept or re-run, lanes={lanes}")
+    rng = np.random.default_rng(43 + lanes)
+    pairs = []
+    for _ in range(200):
+        a = W._dna_draw(rng, 300)
+        n = int(rng.integers(150, 261))
+        pre = W._dna_draw(rng, int(rng.integers(lanes * 16, 3 * lanes * 16)))
+        pairs.append((a, np.concatenate([pre, a[20: 20 + n]])))
+    # mismatches and gaps cost enough that the random rows add nothing
+    batch = W.make_batch(pairs, "lastfall", dict(W.DNA_SW, matchValue=20, mismatchValue=-20,
+                                                 ambiguityValue=-20, gapInit=-20, gapExtend=-5))
+    eng = engine_for(batch["sw"], False)
+    eng.set_lanes_per_pair(lanes)
+    got = device_run(eng, batch)
+    assert_same(got, oracle_batch(batch, False), f"last strip tag fallback, lanes={lanes}")
+    assert int(got[0].min()) < 3500 and int(got[0].max()) > 4500
+
+
 @pytest.mark.parametrize("forward_only", [True, False])
 def test_int32_rerun_long_pairs(gpu, forward_only):
     """Long saturating pairs (2-4 kb, every score above 32,767) through the
