@@ -102,7 +102,8 @@ struct BucketCtl {
   int32_t run_group_base[kRuns + 1];  // first group of each run; [kRuns] = total
   int32_t ngroups;
   int32_t error;       // bit 0: a length outside [0, kMaxSeqLen]; bit 1: an end
-                       // position outside its sequence; bit 2: a bad group
+                       // position outside its sequence; bit 2: a bad group; bit 3: no
+                       // free band slot; bit 4: an int32 boundary-row wait timed out
   int32_t ovf_count;   // int16-saturated comparisons listed for int32
   int32_t maxm;        // longest A of the batch
   int32_t maxn;        // longest B of the batch

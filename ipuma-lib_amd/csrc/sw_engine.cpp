@@ -653,6 +653,7 @@ int error_from_bits(int32_t err) {
   if (err & 2) return fail(IPUMA_EDEVICE, "internal: an end position outside its sequence");
   if (err & 4) return fail(IPUMA_EDEVICE, "internal: a work-group descriptor out of range");
   if (err & 8) return fail(IPUMA_EDEVICE, "internal: no free band slot");
+  if (err & 16) return fail(IPUMA_EDEVICE, "internal: an int32 boundary-row wait timed out");
   return IPUMA_OK;
 }
 

@@ -1134,26 +1134,59 @@ template __global__ void k_fast<26, true>(FastArgs);
 // ---------------------------------------------------------------------------
 constexpr int kW32Rows = 16;
 static_assert(kW32Rows <= 16, "k_wave32 row keys hold the row in 4 bits");
+// Waves of a block: they share one comparison at a time, super-strip s on
+// wave s % kW32Waves (three waves fill a SIMD's issue slots where one alone
+// waits on its own dependency chains).
+constexpr int kW32Waves = 3;
+// Sleeps after which a wait for the boundary row is taken as lost (about a
+// second; every wait ends long before in a working kernel): the kernel then
+// reports error bit 16 and stops waiting.
+constexpr int kW32SpinLimit = 1 << 24;
 
-__global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
+__global__ __launch_bounds__(64 * kW32Waves) void k_wave32(GenericArgs G) {
   constexpr int R = kW32Rows;
+  constexpr int NW = kW32Waves;
   __shared__ int32_t tab[26 * 26];
-  const int lane = threadIdx.x;
+  // per wave: (super-strip << 16) + boundary columns its current super-strip
+  // has written (monotone over a comparison)
+  __shared__ int prog[NW];
+  __shared__ int res[NW][3];  // per wave: its best (H, row, column) of the comparison
+  __shared__ int lost;        // a wait timed out: stop waiting
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int kp = G.kp;
-  for (int i = lane; i < kp * kp; i += 64) tab[i] = G.table[i];
+  for (int i = threadIdx.x; i < kp * kp; i += blockDim.x) tab[i] = G.table[i];
+  if (threadIdx.x == 0) lost = 0;
   __syncthreads();
-  // boundary row of the wave: (H, F) per column
+  // boundary row of the block's comparison: (H, F) per column, updated in
+  // place (super-strip s reads column c a block of 64 columns before its step
+  // c and overwrites it at step c + 63)
   int32_t *const bnd = G.work + (int64_t)blockIdx.x * G.work_stride;
   const int total = G.list ? min(*G.count, G.n) : G.n;
   const int gI = G.gap_init, gE = G.gap_extend, gIE = gI + gE;
   const bool pen = gIE <= 0 && gE <= 0;
   const bool first = lane == 0, last = lane == 63;
+  // Waits until the previous super-strip's wave has published `need`; a lost
+  // wait (a kernel fault, never a working schedule) is reported, not hung on.
+  auto wait_prog = [&](int *pr, int need) {
+    int v = lds_load_acq(pr);
+    for (int spins = 0; v < need; ++spins) {
+      if (spins >= kW32SpinLimit || lds_load_acq(&lost)) {
+        // every lane: lane 0 carries the error bit (no lane-conditional branch)
+        atomicOr(G.error, lane == 0 ? 16 : 0);
+        __hip_atomic_store(&lost, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return;
+      }
+      __builtin_amdgcn_s_sleep(2);
+      v = lds_load_acq(pr);
+    }
+  };
   for (int idx = blockIdx.x; idx < total; idx += gridDim.x) {
     const int p = G.list ? G.list[idx] : idx;
     const int m = G.a_len[p], n = G.b_len[p];
     if (m < 0 || n < 0 || m > G.max_a || m > kMaxSeqLen || n > kMaxSeqLen ||
         (int64_t)2 * m > G.work_stride) {
-      if (first) {
+      if (threadIdx.x == 0) {
         atomicOr(G.error, 1);
         G.scores[p] = -1;
         G.a_range[p] = 0;
@@ -1162,7 +1195,7 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
       continue;
     }
     if (m == 0 || n == 0) {
-      if (first) {
+      if (threadIdx.x == 0) {
         G.scores[p] = 0;
         G.a_range[p] = 0;
         G.b_range[p] = 0;
@@ -1173,13 +1206,16 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
     const uint8_t *b = G.codes + G.b_off[p];
     int best = 0, bi = 0, bj = 0;
     const int nss = (n + 64 * R - 1) / (64 * R);
-    for (int ss = 0; ss < nss; ++ss) {
-      // lane 63's boundary stores of the previous super-strip (the last one
-      // issued in its final step) visible to every lane's loads below
-      if (ss > 0) {
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-      }
+    if (threadIdx.x < NW) prog[threadIdx.x] = -1;
+    __syncthreads();
+    // Super-strip ss runs on wave ss % NW after the wave's earlier ones; it
+    // waits only for super-strip ss - 1 (on another wave), which waits only
+    // for earlier ones: every wait ends.  Its publications grow over the
+    // comparison, so a later super-strip's value also satisfies a wait.
+    for (int ss = wv; ss < nss; ss += NW) {
+      int *const prev = &prog[(ss + NW - 1) % NW];
+      int *const mine = &prog[wv];
+      const int pbase = (ss - 1) << 16;
       const int i0 = (ss * 64 + lane) * R;
       const bool load_bnd = ss > 0, store_bnd = ss + 1 < nss;
       // Gaps that cost (gI + gE <= 0, gE <= 0) bound every H by |A| * 127 <
@@ -1219,9 +1255,13 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
         auto load_block = [&](int c0, int &av, int &hv, int &fv) {
           const int c = c0 + lane;
           av = c < m ? min((int)a[c], kp - 1) * kp : 0;
-          if (load_bnd && c < m) {
-            hv = bnd[2 * c];
-            fv = bnd[2 * c + 1];
+          if (load_bnd) {
+            // the 64 columns from c0 written by the previous super-strip
+            if (c0 < m) wait_prog(prev, pbase + min(m, c0 + 64));
+            if (c < m) {
+              hv = bnd[2 * c];
+              fv = bnd[2 * c + 1];
+            }
           }
         };
         load_block(0, a_cur, h_cur, f_cur);
@@ -1294,6 +1334,16 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
             }
             hdiag = up_h;
           }
+          // publish the boundary columns written so far (lane 63 wrote
+          // column t - 63 this step)
+          if (store_bnd && (t & 15) == 15) {
+            __builtin_amdgcn_wave_barrier();
+            lds_store_rel(mine, (ss << 16) + max(0, min(m, t - 62)));
+          }
+        }
+        if (store_bnd) {
+          __builtin_amdgcn_wave_barrier();
+          lds_store_rel(mine, (ss << 16) + m);
         }
         // rows in order, strict >: the lane's first maximum in B order so far
         if constexpr (PEN) {
@@ -1315,7 +1365,7 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
       if (pen) run(std::true_type{});
       else run(std::false_type{});
     }
-    // lanes hold disjoint rows: (max H, min row)
+    // lanes and waves hold disjoint rows: (max H, min row)
     for (int off = 1; off < 64; off <<= 1) {
       const int oh = __shfl_xor(best, off), oi = __shfl_xor(bi, off), oj = __shfl_xor(bj, off);
       if (oh > best || (oh == best && oi < bi)) {
@@ -1325,10 +1375,23 @@ __global__ __launch_bounds__(64) void k_wave32(GenericArgs G) {
       }
     }
     if (first) {
+      res[wv][0] = best;
+      res[wv][1] = bi;
+      res[wv][2] = bj;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < NW; ++w)
+        if (res[w][0] > best || (res[w][0] == best && res[w][1] < bi)) {
+          best = res[w][0];
+          bi = res[w][1];
+          bj = res[w][2];
+        }
       G.scores[p] = best;
       G.a_range[p] = (int32_t)((uint32_t)(bj & 0xFFFF) << 16);
       G.b_range[p] = (int32_t)((uint32_t)(bi & 0xFFFF) << 16);
     }
+    __syncthreads();  // prog and res serve the next comparison
   }
 }
 
@@ -2136,7 +2199,7 @@ int fast_blocks_per_cu(int kp) {
 }
 
 hipError_t launch_generic(const GenericArgs &G, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_wave32, dim3(blocks), dim3(64), 0, st, G);
+  hipLaunchKernelGGL(k_wave32, dim3(blocks), dim3(64 * kW32Waves), 0, st, G);
   return hipGetLastError();
 }
 
