@@ -1155,6 +1155,8 @@ __global__ __launch_bounds__(64 * kW32Waves) void k_wave32(GenericArgs G) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int kp = G.kp;
+  const int total = G.list ? min(*G.count, G.n) : G.n;
+  if ((int)blockIdx.x >= total) return;  // uniform over the block: no comparison for it
   for (int i = threadIdx.x; i < kp * kp; i += blockDim.x) tab[i] = G.table[i];
   if (threadIdx.x == 0) lost = 0;
   __syncthreads();
@@ -1162,7 +1164,6 @@ __global__ __launch_bounds__(64 * kW32Waves) void k_wave32(GenericArgs G) {
   // place (super-strip s reads column c a block of 64 columns before its step
   // c and overwrites it at step c + 63)
   int32_t *const bnd = G.work + (int64_t)blockIdx.x * G.work_stride;
-  const int total = G.list ? min(*G.count, G.n) : G.n;
   const int gI = G.gap_init, gE = G.gap_extend, gIE = gI + gE;
   const bool pen = gIE <= 0 && gE <= 0;
   const bool first = lane == 0, last = lane == 63;
@@ -1474,19 +1475,25 @@ __device__ __forceinline__ int mbin_wave_of(int m) {
 
 // Device-scope atomics on one address serialize across the XCDs, so the
 // batch-wide sums below are reduced per block first: one atomic per block.
-__global__ __launch_bounds__(256) void k_sum_cells(int32_t n, const int32_t *a_len,
-                                                   const int32_t *b_len, BucketCtl *ctl) {
-  __shared__ unsigned long long s_c[4];
-  __shared__ int s_m[4], s_n[4];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// Batch totals (cells, longest A and B): kSumBlocks blocks at most.
+constexpr int kSumThreads = 1024;
+constexpr int kSumBlocks = 64;
+__global__ __launch_bounds__(kSumThreads) void k_sum_cells(int32_t n, const int32_t *a_len,
+                                                           const int32_t *b_len, BucketCtl *ctl) {
+  __shared__ unsigned long long s_c[kSumThreads / 64];
+  __shared__ int s_m[kSumThreads / 64], s_n[kSumThreads / 64];
   const int w = threadIdx.x >> 6;
   unsigned long long c = 0;
   int m = 0, nb = 0;
-  if (p < n) {
-    m = a_len[p];
-    nb = b_len[p];
-    if (m > 0 && nb > 0 && m <= kMaxSeqLen && nb <= kMaxSeqLen) c = (unsigned long long)m * nb;
-    else m = nb = 0;
+  // grid-stride: a few blocks, so that few device atomics hit the three
+  // counters (atomics on one address serialize across the chip)
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const int pm = a_len[p], pn = b_len[p];
+    if (pm > 0 && pn > 0 && pm <= kMaxSeqLen && pn <= kMaxSeqLen) {
+      c += (unsigned long long)pm * pn;
+      m = max(m, pm);
+      nb = max(nb, pn);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     c += __shfl_xor(c, off);
@@ -2127,8 +2134,9 @@ hipError_t launch_encode(const uint8_t *chars, int64_t len, uint8_t *codes, cons
 
 hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
   const int n = B.n;
-  const int nb = nblocks(std::max(n, 1), 256);
-  hipLaunchKernelGGL(k_sum_cells, dim3(nb), dim3(256), 0, st, n, B.a_len, B.b_len, B.ctl);
+  const int nb = nblocks(std::max(n, 1), 256);  // one thread per pair, 256 per block
+  hipLaunchKernelGGL(k_sum_cells, dim3(std::min(nblocks(std::max(n, 1), kSumThreads), kSumBlocks)),
+                     dim3(kSumThreads), 0, st, n, B.a_len, B.b_len, B.ctl);
   hipLaunchKernelGGL(k_classify, dim3(nblocks(std::max(n, 1), kClsThreads)), dim3(kClsThreads), 0,
                      st, n, B.a_len, B.b_len, B.target_slots, B.plain ? 0 : B.force_class,
                      B.wave_cols, B.max_a, B.bin_of, B.rank, B.hist, B.ctl, B.scores, B.a_range, B.b_range,

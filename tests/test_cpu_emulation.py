@@ -383,6 +383,24 @@ def test_last_strip_optimistic_tags_emulated(emu_lib):
     eng.close()
 
 
+def test_many_pairs_emulated(emu_lib):
+    """3,000 short pairs (more than one block of every bucketing kernel: the
+    batch totals, classification, scatter), each result against the oracle."""
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(88)
+    A = ["".join(rng.choice(list("ACGT"), int(rng.integers(3, 24)))) for _ in range(3000)]
+    B = ["".join(rng.choice(list("ACGT"), int(rng.integers(3, 24)))) for _ in range(3000)]
+    exp = O.sw_batch(A, B, gap_init=-2, gap_extend=-1, forward_only=True)
+    eng = I.SWAlgorithm(I.SWConfig(gapInit=-2, gapExtend=-1),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1,
+                                        forwardOnly=True))
+    got = eng.compare(A, B)
+    for g, e, nm in zip(got, exp, ("scores", "a_range", "b_range")):
+        assert np.array_equal(np.asarray(g), np.asarray(e)), nm
+    eng.close()
+
+
 def _tie_pair(rng, n):
     """A = X + Y, B = Y + X with |X| = |Y| = n: two equal best alignments, the
     one on the earlier B row ending at the later A column."""
