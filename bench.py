@@ -24,6 +24,8 @@ with its own parity flags and roofline (packed-kernel time from HIP events):
   int32_rerun  1,000 DNA pairs of 2-4 kb at 95% identity scored +20 / -20,
             gap open 20 / extend 2, whose scores pass the int16 range, so
             every pair is re-run exactly in int32;
+  traceback  CIGARs of 20,000 protein pairs from a compare with starts
+            (k_trace_wave, a wave per comparison, int32);
 and ``serving``: Poisson arrivals at 50, 200 and 1,000 req/s (shortened to
 --serve-seconds each), merged batches and one launch per request.
 ``valu_peak`` is the measured packed-int16 / int32 VALU throughput the
@@ -46,7 +48,7 @@ OPS_PER_CELL = 11          # E: 2 add + max, F: 2 add + max, H: add + 3 max, bes
 PEAK_INT16_TOPS = 78.6     # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz x 2 (packed int16)
 PEAK_INT32_TOPS = 39.3     # the same, one op per lane
 METRIC = "GCUPS on 1 MI355X (bucketed vs plain, bit-exact); p50/p99 latency at fixed req/s"
-ALL_SUBCONFIGS = ("dna1024", "skewed", "search", "int32_rerun")
+ALL_SUBCONFIGS = ("dna1024", "skewed", "search", "int32_rerun", "traceback")
 
 
 def log(*a):
@@ -533,6 +535,77 @@ def bench_rerun32(args, peak_cal):
     return rec
 
 
+def bench_traceback(args, peak_cal):
+    """CIGARs of 20,000 protein pairs (the protein batch's generator) from a compare with
+    starts: host characters in, runs out (ipuma_traceback_local), k_trace_wave timed with
+    HIP events; 300 CIGARs against the oracle."""
+    import numpy as np
+    import ipuma_amd as I
+    from ipuma_amd import workloads as W
+    b = W.config1(n=20_000)
+    letters = np.frombuffer(b"ARNDCQEGHILKMFPSTWYVBZX*", np.uint8)
+    codes = b["codes"]
+    A = [letters[codes[o:o + n]].tobytes() for o, n in zip(b["a_off"], b["a_len"])]
+    B = [letters[codes[o:o + n]].tobytes() for o, n in zip(b["b_off"], b["b_len"])]
+    eng = I.SWAlgorithm(I.SWConfig(**b["sw"]),
+                        I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    s, ar, br = eng.compare(A, B)
+    n = len(A)
+    abuf, aoff = I._pack(A)
+    bbuf, boff = I._pack(B)
+    u = ar.astype(np.int64) & 0xFFFFFFFF
+    v = br.astype(np.int64) & 0xFFFFFFFF
+    cap = np.where(s > 0, ((u >> 16) - (u & 0xFFFF) + 1) + ((v >> 16) - (v & 0xFFFF) + 1), 0)
+    offs = np.zeros(n + 1, np.int64)
+    offs[1:] = np.cumsum(cap)
+    cig = np.zeros(max(int(offs[-1]), 1), np.uint32)
+    lens = np.zeros(n, np.int32)
+    L = I.lib()
+
+    def call():
+        I._check(L.ipuma_traceback_local(eng._h, n, abuf, aoff.ctypes.data, bbuf, boff.ctypes.data,
+                                         s.ctypes.data, ar.ctypes.data, br.ctypes.data,
+                                         cig.ctypes.data, offs.ctypes.data, lens.ctypes.data))
+
+    eng.set_timing(True)
+    call()
+    steps = max(3, args.sub_steps)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+        kms.append(eng.traceback_stats()["kernel_ms"])
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    cells = eng.traceback_stats()["region_cells"]
+    O = oracle()
+    mat = matrix_of(O, b["sw"])
+    rng = np.random.default_rng(3)
+    ok = True
+    for k in rng.choice(n, 300, replace=False):
+        a = codes[b["a_off"][k]:b["a_off"][k] + b["a_len"][k]]
+        bb = codes[b["b_off"][k]:b["b_off"][k] + b["b_len"][k]]
+        exp = O.cigar(a, bb, int(ar[k]), int(br[k]), int(s[k]), mat, b["sw"]["gapInit"],
+                      b["sw"]["gapExtend"])
+        ok &= bool(np.array_equal(cig[offs[k]:offs[k] + lens[k]], exp))
+    eng.close()
+    kbest = min(kms)
+    rec = {"workload": "traceback (CIGAR) of 20,000 protein pairs of the protein batch's "
+                       "generator from a compare with starts, BLOSUM62, gap open 11 extend 1",
+           "pairs": n, "region_cells": cells, "steps": steps,
+           "ms_per_call": round(ms, 3),
+           "gcups_call": round(cells / ms / 1e6, 1),
+           "kernel_ms": round(float(np.median(kms)), 3), "kernel_ms_min": round(kbest, 3),
+           "gcups_kernel": round(cells / float(np.median(kms)) / 1e6, 1),
+           "what": "ms_per_call: ipuma_traceback_local with host characters in and runs out "
+                   "(staging, kernel, copy back); kernel: k_trace_wave, HIP events",
+           "parity": {"oracle_cigar_300_pairs": ok},
+           "roofline": roofline(cells, float(np.median(kms)), "k_trace_wave (HIP events)",
+                                getattr(args, "pk32", None), precision="int32")}
+    log(f"traceback: {rec['ms_per_call']} ms/call, kernel {rec['kernel_ms']} ms, "
+        f"{rec['gcups_kernel']} GCUPS (kernel), parity {rec['parity']}")
+    return rec
+
+
 def bench_serving(args):
     from ipuma_amd import loadgen
     log("serving: generating request pools ...")
@@ -658,7 +731,7 @@ def main():
     configs = {}
     for name in [c for c in args.subconfigs.split(",") if c]:
         fn = {"dna1024": bench_dna1024, "skewed": bench_skewed, "search": bench_search,
-              "int32_rerun": bench_rerun32}[name]
+              "int32_rerun": bench_rerun32, "traceback": bench_traceback}[name]
         try:
             configs[name] = fn(args, pk16)
         except Exception as e:  # reported in the line; the headline stands

@@ -260,6 +260,13 @@ int ipuma_traceback_local(ipuma_engine *e, int32_t n, const char *a_chars,
                           const int64_t *a_offsets, const char *b_chars, const int64_t *b_offsets,
                           const int32_t *scores, const int32_t *a_range, const int32_t *b_range,
                           uint32_t *cigar, const int64_t *cigar_offsets, int32_t *cigar_len);
+/* Of the last ipuma_traceback_local call: its region cells (sum of
+ * (Aend - Astart + 1) * (Bend - Bstart + 1) over comparisons with a positive
+ * score) and, with timing on (ipuma_set_timing), the time of its traceback
+ * kernels (HIP events around each launch on the engine's stream, summed), else
+ * 0.  Either pointer may be NULL.  No reference counterpart: the traceback's
+ * throughput. */
+int ipuma_last_traceback_stats(ipuma_engine *e, int64_t *region_cells, double *kernel_ms);
 
 /* Batching policy of the device path (replaces IPUAlgoConfig.fillAlgo's
  * partition::fillFirst/roundRobin/greedy, src/partition.cpp:197-462). */

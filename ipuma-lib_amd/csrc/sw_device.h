@@ -28,6 +28,13 @@ constexpr int64_t kWaveScratchWords = 3 * 65536 + 320;
 constexpr int64_t kBandScratchWords = 2 * 65536;
 // Longest sequence the engine accepts (positions are packed as uint16).
 constexpr int32_t kMaxSeqLen = 65535;
+// Rows per lane of k_trace_wave: a strip is 64 * R rows, and a lane's
+// direction bits (4 per cell) take one 32-bit word per step.  R is the fewest
+// of 2, 4, 6, 8 whose strip holds the region's |A| rows, else kTrRows.
+constexpr int kTrRows = 8;
+__host__ __device__ constexpr int trace_rows(int64_t m) {
+  return m <= 128 ? 2 : m <= 256 ? 4 : m <= 384 ? 6 : kTrRows;
+}
 // Counting-sort bins of the length bucketing: class x super-strip bin x
 // (rows of the last super-strip, m-bin).  A run is one (class, super-strip
 // bin); groups never straddle runs, and each run starts at a multiple of
@@ -194,6 +201,8 @@ struct TraceArgs {
   int64_t *work;
   uint32_t *cigar;          // runs, end to start
   int32_t *cig_len;         // [n] runs, or -2 when the region's optimum differs from the score
+  const int32_t *order;     // [n] comparison of each wave (k_trace_wave), or null
+  int32_t wave;             // 1: k_trace_wave (int32, a wave per comparison), 0: k_traceback
 };
 
 // Arguments of the bucketing pipeline.

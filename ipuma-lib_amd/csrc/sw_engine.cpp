@@ -175,6 +175,9 @@ struct ipuma_engine {
   double w_h2d_ms = 0.0, w_device_ms = 0.0;          // their intervals for the last prepared batch
   // traceback
   DevBuf t_desc, t_dir, t_work, t_cig;
+  hipEvent_t t_ev[2] = {nullptr, nullptr};  // around each traceback launch (timing on)
+  int64_t t_cells = 0;                       // region cells of the last traceback call
+  double t_kernel_ms = 0.0;                  // its kernels' time (timing on)
   std::vector<int64_t> h_src, h_dst;
   std::vector<int32_t> h_len;
   // executor thread of asynchronously submitted prepared batches
@@ -970,6 +973,8 @@ void ipuma_engine_destroy(ipuma_engine *e) {
     if (ev) (void)hipEventDestroy(ev);
   for (auto &ev : e->w_ev)
     if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : e->t_ev)
+    if (ev) (void)hipEventDestroy(ev);
   if (e->d_table) (void)hipFree(e->d_table);
   if (e->d_codetab) (void)hipFree(e->d_codetab);
   if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1210,6 +1215,14 @@ int ipuma_set_forward_only(ipuma_engine *e, int32_t forward_only) {
   return IPUMA_OK;
 }
 
+int ipuma_last_traceback_stats(ipuma_engine *e, int64_t *region_cells, double *kernel_ms) {
+  if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
+  std::lock_guard<std::mutex> lock_(e->mu);
+  if (region_cells) *region_cells = e->t_cells;
+  if (kernel_ms) *kernel_ms = e->t_kernel_ms;
+  return IPUMA_OK;
+}
+
 int ipuma_set_timing(ipuma_engine *e, int32_t on) {
   if (!e) return fail(IPUMA_EINVAL, "engine is NULL");
   std::lock_guard<std::mutex> lock_(e->mu);
@@ -1355,6 +1368,16 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
                 !cigar_offsets || !cigar_len))
     return fail(IPUMA_EINVAL, "NULL buffer");
   std::lock_guard<std::mutex> lock_(e->mu);
+  // k_trace_wave (int32) when every region's values are bounded by 2^28: a
+  // path of the region has at most m + n steps, each worth at most a gap
+  // open, an extension and a score in magnitude (one pad cell adds 2^15)
+  const int64_t go = -(int64_t)(e->sw.gap_init + e->sw.gap_extend), ge = -(int64_t)e->sw.gap_extend;
+  int64_t sabs = 0;
+  for (int x = 0; x + 1 < e->kp; ++x)
+    for (int y = 0; y + 1 < e->kp; ++y)
+      sabs = std::max<int64_t>(sabs, std::abs((int64_t)e->h_table[(size_t)x * e->kp + y]));
+  const int64_t per_step = std::abs(go) + std::abs(ge) + sabs;
+  bool wave = true;
   // regions, capacities, scratch offsets
   std::vector<const char *> ptrs(2 * (size_t)n);
   std::vector<int64_t> lens(2 * (size_t)n);
@@ -1384,18 +1407,33 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
     if (cap < m + nb)
       return fail(IPUMA_EINVAL, "comparison %d: CIGAR capacity %lld < %lld", k, (long long)cap,
                   (long long)(m + nb));
+    if ((m + nb + 2) * per_step + 65536 > ((int64_t)1 << 28)) wave = false;
     dir_off[k] = m * nb;   // sizes for now; turned into offsets per chunk below
     work_off[k] = 2 * (nb + 1);
     cig_off[k] = m + nb;
     need_cig += m + nb;
   }
   (void)need_cig;
+  e->t_cells = 0;
+  e->t_kernel_ms = 0.0;
+  for (int32_t k = 0; k < n; ++k) e->t_cells += dir_off[k];  // region cells (sizes not yet replaced)
+  if (e->timing && !e->t_ev[0])
+    for (auto &ev : e->t_ev) HIP_TRY(hipEventCreate(&ev));
+  if (wave)  // a 32-bit word per lane and step: strips of 64 * R A rows x (|B region| + 63) steps x 256 B
+    for (int32_t k = 0; k < n; ++k) {
+      if (!dir_off[k]) continue;
+      const int64_t cols = work_off[k] / 2 - 1;  // |B region|
+      const int64_t rows = dir_off[k] / cols, strip = 64 * trace_rows(rows);
+      dir_off[k] = (rows + strip - 1) / strip * (cols + 63) * 256;
+    }
   std::vector<int64_t> offs;
   int64_t total = 0;
   int rc;
   if ((rc = stage_arena(e, ptrs, lens, offs, total))) return rc;
-  // chunks of at most 1 GiB of direction cells (one pair may exceed it alone)
-  const int64_t kChunkDir = (int64_t)1 << 30;
+  // chunks of at most 8 GiB of direction storage (one pair may exceed it
+  // alone): one launch for most calls, so that the longest regions, which
+  // start first, overlap every other comparison of the call
+  const int64_t kChunkDir = (int64_t)8 << 30;
   std::vector<int64_t> h_desc;
   std::vector<uint32_t> h_cig;
   std::vector<int32_t> h_len;
@@ -1422,7 +1460,13 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
       wpos += work_off[c0 + k];
       cpos += cig_off[c0 + k];
     }
-    if ((rc = e->t_desc.ensure((size_t)cn * (5 * 8 + 3 * 4 + 4))) ||
+    // k_trace_wave: largest regions first (one wave each)
+    std::vector<int32_t> order(cn);
+    for (int32_t k = 0; k < cn; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+      return dir_off[c0 + x] > dir_off[c0 + y];
+    });
+    if ((rc = e->t_desc.ensure((size_t)cn * (5 * 8 + 4 * 4 + 4))) ||
         (rc = e->t_dir.ensure((size_t)std::max<int64_t>(dsum, 1))) ||
         (rc = e->t_work.ensure((size_t)std::max<int64_t>(wsum, 1) * 8)) ||
         (rc = e->t_cig.ensure((size_t)std::max<int64_t>(csum, 1) * 4)))
@@ -1434,12 +1478,16 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
     HIP_TRY(hipMemcpyAsync(d32 + cn, b_range + c0, (size_t)cn * 4, hipMemcpyHostToDevice, e->stream));
     HIP_TRY(hipMemcpyAsync(d32 + 2 * (size_t)cn, scores + c0, (size_t)cn * 4, hipMemcpyHostToDevice,
                            e->stream));
+    HIP_TRY(hipMemcpyAsync(d32 + 4 * (size_t)cn, order.data(), (size_t)cn * 4, hipMemcpyHostToDevice,
+                           e->stream));
     TraceArgs T{};
     T.codes = e->in_codes.as<uint8_t>();
     T.a_off = d64; T.b_off = d64 + cn; T.dir_off = d64 + 2 * (size_t)cn;
     T.work_off = d64 + 3 * (size_t)cn; T.cig_off = d64 + 4 * (size_t)cn;
     T.a_range = d32; T.b_range = d32 + cn; T.scores = d32 + 2 * (size_t)cn;
     T.cig_len = d32 + 3 * (size_t)cn;
+    T.order = d32 + 4 * (size_t)cn;
+    T.wave = wave ? 1 : 0;
     T.n = cn;
     T.gap_o = -(e->sw.gap_init + e->sw.gap_extend);
     T.gap_x = -e->sw.gap_extend;
@@ -1447,13 +1495,20 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
     T.dir = e->t_dir.as<uint8_t>(); T.work = e->t_work.as<int64_t>();
     T.cigar = e->t_cig.as<uint32_t>();
     if ((rc = begin_work(e, e->stream))) return rc;
+    if (e->timing) HIP_TRY(hipEventRecord(e->t_ev[0], e->stream));
     HIP_TRY(launch_traceback(T, e->stream));
+    if (e->timing) HIP_TRY(hipEventRecord(e->t_ev[1], e->stream));
     if ((rc = end_work(e, e->stream))) return rc;
     h_cig.resize((size_t)std::max<int64_t>(csum, 1));
     h_len.resize(cn);
     HIP_TRY(hipMemcpyAsync(h_cig.data(), T.cigar, (size_t)csum * 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipMemcpyAsync(h_len.data(), T.cig_len, (size_t)cn * 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->timing) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, e->t_ev[0], e->t_ev[1]));
+      e->t_kernel_ms += ms;
+    }
     for (int32_t k = 0; k < cn; ++k) {
       const int32_t r = h_len[k];
       cigar_len[c0 + k] = r;

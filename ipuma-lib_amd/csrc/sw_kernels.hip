@@ -2075,6 +2075,222 @@ __global__ __launch_bounds__(64) void k_traceback(TraceArgs T) {
   T.cig_len[p] = runs;
 }
 
+// The same traceback on one wave per comparison, in int32 (the host takes it
+// when every value of the call's regions is bounded well inside int32: the
+// boundary value kTrNeg and its differences with real values then stay in
+// range, and every comparison decides as the int64 kernel's).  The region's
+// rows (A) run in strips of 64 * R (R = trace_rows(|A region|): 2, 4, 6 or 8
+// rows per lane, the fewest that cover a region of up to 512 rows in one
+// strip); lane k owns rows k * R .. + R - 1 of a strip and at step t computes
+// column t - k of them,
+// receiving H and E of the row above (and the column's B code) from lane
+// k - 1 through DPP; lane 0 reads the row above the strip from the
+// comparison's work area, where lane 63 wrote it during the previous strip.
+// Four direction bits per cell, one 32-bit word per lane and step, stored
+// coalesced at dir[(strip * (n + 63) + t) * 64 + lane]: row r's nibble at
+// bit 4 * (R - 1 - r), its bits 3 (H not from the diagonal), 2 (E above F),
+// 1 (E opened) and 0 (F opened).  The wave then walks back as k_traceback does.
+constexpr int kTrNeg = -(1 << 30);
+
+template <int R>
+__device__ __forceinline__ void trace_wave(const TraceArgs &T, const int32_t *tab, int p, int s, int as, int bs,
+                           int m, int n) {
+  const int lane = threadIdx.x;
+  const int kp = T.kp;
+  const uint8_t *a = T.codes + T.a_off[p] + as;
+  const uint8_t *b = T.codes + T.b_off[p] + bs;
+  uint32_t *const dirw = reinterpret_cast<uint32_t *>(T.dir + T.dir_off[p]);
+  int32_t *const bnd = reinterpret_cast<int32_t *>(T.work + T.work_off[p]);
+  const int go = T.gap_o, ge = T.gap_x;
+  const int nss = (m + 64 * R - 1) / (64 * R);
+  const int TT = n + 63;  // steps per strip
+  const bool first = lane == 0, last = lane == 63;
+  int Hc[R];
+  for (int ss = 0; ss < nss; ++ss) {
+    // the lane's rows: 0-based A index i0 + r, DP row i0 + r + 1
+    const int i0 = (ss * 64 + lane) * R;
+    int arow[R], F[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = i0 + r;
+      arow[r] = (i < m ? min((int)a[i], kp - 1) : 0) * kp;
+      Hc[r] = -(go + i * ge);  // H(i + 1, 0)
+      F[r] = kTrNeg;           // F(i + 1, 0)
+    }
+    int hdiag = i0 == 0 ? 0 : -(go + (i0 - 1) * ge);  // H(i0, 0)
+    // lane 0's inputs at step t (column t's B code, and H and E of the row
+    // above the strip) come from blocks of 64 columns that every lane loads
+    // one column of, a block ahead, read with v_readlane; lane 63 overwrites
+    // column c of the work area at step c + 63, after c's block was loaded
+    auto load_block = [&](int c0, int &bv, int &hv, int &ev) {
+      const int c = c0 + lane;
+      bv = c < n ? min((int)b[c], kp - 1) : 0;
+      if (ss == 0) {
+        hv = -(go + c * ge);  // H(0, c + 1)
+        ev = kTrNeg;          // E(0, c + 1)
+      } else if (c < n) {
+        hv = bnd[2 * c];
+        ev = bnd[2 * c + 1];
+      }
+    };
+    int b_cur, h_cur = 0, e_cur = 0, b_nxt, h_nxt = 0, e_nxt = 0;
+    load_block(0, b_cur, h_cur, e_cur);
+    load_block(64, b_nxt, h_nxt, e_nxt);
+    int out_h = 0, out_e = 0, col = 0;
+    uint32_t *const dws = dirw + (int64_t)ss * TT * 64 + lane;
+    const bool store_bnd = last && ss + 1 < nss;
+    for (int t = 0; t < TT; ++t) {
+      const int j = t - lane;  // 0-based B index: DP column j + 1
+      if (t > 0 && (t & 63) == 0) {
+        b_cur = b_nxt;
+        h_cur = h_nxt;
+        e_cur = e_nxt;
+        load_block(t + 64, b_nxt, h_nxt, e_nxt);
+      }
+      const int sl = t & 63;
+      const int bc = __builtin_amdgcn_readlane(b_cur, sl);
+      const int bh = __builtin_amdgcn_readlane(h_cur, sl);
+      const int bee = __builtin_amdgcn_readlane(e_cur, sl);
+      const int up_h = (int)from_prev<kModeWave>((uint32_t)bh, (uint32_t)out_h, first);
+      const int up_e = (int)from_prev<kModeWave>((uint32_t)bee, (uint32_t)out_e, first);
+      col = (int)from_prev<kModeWave>((uint32_t)bc, (uint32_t)col, first);
+      if (j >= 0 && j < n) {
+        int hd = hdiag, hup = up_h, eup = up_e;
+        uint32_t w = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int sc = tab[arow[r] + col];
+          const int eop = hup - go, eex = eup - ge;
+          const int e = max(eop, eex);
+          const int fop = Hc[r] - go, fex = F[r] - ge;
+          const int f = max(fop, fex);
+          const int d = hd + sc;
+          const int h = max(d, max(e, f));
+          // each bit is the sign of a difference, shifted in from the right
+          w = __builtin_amdgcn_alignbit(w, (uint32_t)(d - h), 31);      // d < h
+          w = __builtin_amdgcn_alignbit(w, (uint32_t)(f - e), 31);      // e > f
+          w = __builtin_amdgcn_alignbit(w, (uint32_t)(eex - eop), 31);  // E opened
+          w = __builtin_amdgcn_alignbit(w, (uint32_t)(fex - fop), 31);  // F opened
+          hd = Hc[r];
+          Hc[r] = h;
+          F[r] = f;
+          hup = h;
+          eup = e;
+        }
+        out_h = hup;
+        out_e = eup;
+        dws[(int64_t)t * 64] = w;
+        if (store_bnd) {
+          bnd[2 * j] = hup;
+          bnd[2 * j + 1] = eup;
+        }
+        hdiag = up_h;
+      }
+    }
+    __syncthreads();  // the work area and the direction words, for the next strip and the walk
+  }
+  // H(m, n): row m - 1 (0-based) of the last strip, after its last column
+  const int rm = (m - 1) % R, lm = ((m - 1) % (64 * R)) / R;
+  int hv = Hc[0];
+#pragma unroll
+  for (int r = 1; r < R; ++r)
+    if (r == rm) hv = Hc[r];
+  const int hend = __shfl(hv, lm);
+  if (hend != s) {
+    if (first) T.cig_len[p] = -2;
+    return;
+  }
+  // The walk runs on every lane (uniform values); the direction nibbles come
+  // 64 at a time: lane q loads the cell q steps from the walk's position along
+  // its current move (diagonal, up, left), so that one round trip to memory
+  // serves a run of up to 64 moves in the same state.
+  uint32_t *out = T.cigar + T.cig_off[p];
+  int runs = 0, run_op = -1, run_len = 0;
+  int i = m, j = n, st = 0;
+  int pi = -1, pj = -1, pst = -1;  // origin and state of the prefetched line
+  uint32_t pn = 0;                 // lane q: the nibble of the line's cell q
+  while (i > 0 || j > 0) {
+    int op;
+    if (i == 0) {
+      op = 2;
+      --j;
+    } else if (j == 0) {
+      op = 1;
+      --i;
+    } else {
+      const int di = pi - i, dj = pj - j;
+      int q = -1;
+      if (pst == 0 && di == dj && di >= 0 && di < 64) q = di;
+      else if (pst == 1 && dj == 0 && di >= 0 && di < 64) q = di;
+      else if (pst == 2 && di == 0 && dj >= 0 && dj < 64) q = dj;
+      if (q < 0) {
+        pi = i;
+        pj = j;
+        pst = st;
+        const int ci = i - (st != 2 ? lane : 0), cj = j - (st != 1 ? lane : 0);
+        pn = 0;
+        if (ci >= 1 && cj >= 1) {
+          const int ri = ci - 1, li = (ri % (64 * R)) / R;
+          const int64_t idx = ((int64_t)(ri / (64 * R)) * TT + (cj - 1) + li) * 64 + li;
+          pn = (dirw[idx] >> (4 * (R - 1 - ri % R))) & 15u;
+        }
+        q = 0;
+      }
+      const uint32_t nib = (uint32_t)__builtin_amdgcn_readlane((int)pn, q);
+      if (st == 0) st = (nib & 8u) ? ((nib & 4u) ? 1 : 2) : 0;
+      if (st == 0) {
+        op = 0;
+        --i;
+        --j;
+      } else if (st == 1) {
+        op = 1;
+        st = (nib & 2u) ? 0 : 1;
+        --i;
+      } else {
+        op = 2;
+        st = (nib & 1u) ? 0 : 2;
+        --j;
+      }
+    }
+    if (op == run_op) {
+      ++run_len;
+    } else {
+      if (run_len && first) out[runs] = ((uint32_t)run_len << 4) | (uint32_t)run_op;
+      runs += run_len ? 1 : 0;
+      run_op = op;
+      run_len = 1;
+    }
+  }
+  if (run_len && first) out[runs] = ((uint32_t)run_len << 4) | (uint32_t)run_op;
+  runs += run_len ? 1 : 0;
+  if (!first) return;
+  T.cig_len[p] = runs;
+}
+
+__global__ __launch_bounds__(64) void k_trace_wave(TraceArgs T) {
+  __shared__ int32_t tab[26 * 26];
+  const int lane = threadIdx.x;
+  const int kp = T.kp;
+  for (int i = lane; i < kp * kp; i += 64) tab[i] = T.table[i];
+  __syncthreads();
+  const int p = T.order ? T.order[blockIdx.x] : (int)blockIdx.x;
+  const int s = T.scores[p];
+  const uint32_t ar = (uint32_t)T.a_range[p], br = (uint32_t)T.b_range[p];
+  const int as = (int)(ar & 0xFFFF), ae = (int)(ar >> 16);
+  const int bs = (int)(br & 0xFFFF), be = (int)(br >> 16);
+  if (s <= 0 || ae < as || be < bs) {
+    if (lane == 0) T.cig_len[p] = 0;
+    return;
+  }
+  const int m = ae - as + 1, n = be - bs + 1;
+  switch (trace_rows(m)) {
+    case 2: trace_wave<2>(T, tab, p, s, as, bs, m, n); break;
+    case 4: trace_wave<4>(T, tab, p, s, as, bs, m, n); break;
+    case 6: trace_wave<6>(T, tab, p, s, as, bs, m, n); break;
+    default: trace_wave<kTrRows>(T, tab, p, s, as, bs, m, n); break;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Prepared input buffers (the reference's wire format): the residues of each
 // slot's A and B are copied from their tile regions to 4-aligned places of a
@@ -2163,7 +2379,10 @@ hipError_t launch_bucketing(const BucketArgs &B, hipStream_t st) {
 
 hipError_t launch_traceback(const TraceArgs &T, hipStream_t st) {
   if (T.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_traceback, dim3(nblocks(T.n, 64)), dim3(64), 0, st, T);
+  if (T.wave)
+    hipLaunchKernelGGL(k_trace_wave, dim3(T.n), dim3(64), 0, st, T);
+  else
+    hipLaunchKernelGGL(k_traceback, dim3(nblocks(T.n, 64)), dim3(64), 0, st, T);
   return hipGetLastError();
 }
 

@@ -176,6 +176,7 @@ def lib():
     L.ipuma_prepared_submit.argtypes = [_P, _P, _i64, _P, _i64, ctypes.POINTER(_i64)]
     L.ipuma_prepared_join.argtypes = [_P, _i64]
     L.ipuma_traceback_local.argtypes = [_P, _i32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.ipuma_last_traceback_stats.argtypes = [_P, _P, _P]
     return L
 
 
@@ -456,6 +457,14 @@ class SWAlgorithm:
                                            br.ctypes.data, cig.ctypes.data, offs.ctypes.data,
                                            lens.ctypes.data))
         return [cig[offs[k]:offs[k] + lens[k]].copy() for k in range(n)]
+
+    def traceback_stats(self) -> dict:
+        """Region cells of the last traceback call and, with timing on, the time
+        of its traceback kernels in ms (HIP events on the engine's stream)."""
+        cells = ctypes.c_int64(0)
+        ms = ctypes.c_double(0.0)
+        _check(lib().ipuma_last_traceback_stats(self._h, ctypes.byref(cells), ctypes.byref(ms)))
+        return {"region_cells": int(cells.value), "kernel_ms": float(ms.value)}
 
     def get_result(self) -> BlockAlignmentResults:
         """get_result (src/ipu_batch_affine.cpp:408): copies sized tilesUsed*maxBatches."""

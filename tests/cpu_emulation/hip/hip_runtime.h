@@ -145,6 +145,9 @@ void launch(const char *name, K kernel, dim3 grid, dim3 block, Args... args) {
 #define __builtin_amdgcn_update_dpp(o, s, c, r, b, bc) emu::update_dpp((o), (s), (c), (r), (b), (bc))
 #define __builtin_amdgcn_mov_dpp(s, c, r, b, bc) emu::update_dpp(0, (s), (c), (r), (b), (bc))
 #define __builtin_amdgcn_perm(a, b, s) emu::perm((a), (b), (s))
+// v_alignbit_b32: the low 32 bits of ({a, b} >> s)
+#define __builtin_amdgcn_alignbit(a, b, s) \
+  ((uint32_t)(((((uint64_t)(uint32_t)(a)) << 32) | (uint32_t)(b)) >> ((s) & 31)))
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_readfirstlane(v) emu::exchange_get((v), 0)
 // v_readlane: the lane index is uniform over the wave (an SGPR operand)

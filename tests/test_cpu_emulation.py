@@ -551,6 +551,57 @@ def test_traceback_emulated(emu_lib):
         eng.close()
 
 
+def test_traceback_strips_emulated(emu_lib):
+    """The wave traceback over regions of several strips of 512 rows (the row
+    above each strip handed over through the work area) and of one strip of 4
+    and 6 rows per lane, and the int64 kernel that gap settings beyond the
+    int32 bound take: CIGARs equal the oracle's."""
+    import oracle as O
+    I = emu_lib
+    rng = np.random.default_rng(43)
+
+    def mutate(a, letters):
+        out = []
+        for c in a:
+            u = rng.random()
+            if u < 0.08:
+                out.append(rng.choice(list(letters)))
+            elif u < 0.10:
+                continue
+            elif u < 0.12:
+                out.extend([c, rng.choice(list(letters))])
+            else:
+                out.append(c)
+        return "".join(out)
+
+    dna, aa = "ACGT", "ARNDCQEGHILKMFPSTWYV"
+    cases = []
+    # regions of 2-3 strips of 512 rows, and of one strip of 4 and 6 rows per lane
+    A = ["".join(rng.choice(list(dna), n)) for n in (1300, 530, 1030, 200, 350)]
+    cases.append((O.SIM_NA, O.DT_NA, -2, -1, A, [mutate(a, dna) for a in A]))
+    P = ["".join(rng.choice(list(aa), n)) for n in (700, 513, 300, 240)]
+    cases.append((O.SIM_BLOSUM62, O.DT_AA, -10, -1, P, [mutate(a, aa) for a in P]))
+    # a gap open of 20,000,000: every region here (at least 12 cells along its
+    # two sides) is past the int32 bound (2^28), so the int64 kernel runs
+    D = ["".join(rng.choice(list(dna), int(rng.integers(20, 90)))) for _ in range(6)]
+    cases.append((O.SIM_NA, O.DT_NA, -20_000_000, -1, D, [mutate(a, dna) for a in D]))
+    for sim, dt, gi, ge, A, B in cases:
+        sw = I.SWConfig(gapInit=gi, gapExtend=ge, similarity="na" if dt == O.DT_NA else "blosum62",
+                        datatype="na" if dt == O.DT_NA else "aa")
+        eng = I.SWAlgorithm(sw, I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+        s, ar, br = eng.compare(A, B)
+        got = eng.traceback(A, B, s, ar, br)
+        mat = O.select_matrix(sim)
+        for k in range(len(A)):
+            exp = O.cigar(O.encode(dt, A[k]), O.encode(dt, B[k]), int(ar[k]), int(br[k]), int(s[k]),
+                          mat, gi, ge)
+            assert np.array_equal(got[k], exp), (gi, k, I.cigar_string(got[k]), O.cigar_string(exp))
+        if gi == -2:  # the long DNA pairs span two and three strips of 512 rows
+            rows = [((int(ar[k]) >> 16) - (int(ar[k]) & 0xFFFF) + 1) for k in range(len(A))]
+            assert max(rows) > 1024, rows
+        eng.close()
+
+
 def _thread_budget(I):
     import ctypes
     return ctypes.c_int.in_dll(I.lib(), "_ZN5ipuma21g_thread_start_budgetE")

@@ -383,8 +383,8 @@ def test_prepared_buffers(gpu):
 
 def test_traceback(gpu):
     # CIGARs of a compare with starts, bit-exact against the oracle: DNA and
-    # BLOSUM62 batches and a few long pairs (one lane per comparison, chunked
-    # direction storage)
+    # BLOSUM62 batches and a few long pairs (one wave per comparison, regions
+    # of one to ten strips, 2 to 8 rows per lane)
     note("traceback: 3000 DNA + 1500 protein pairs + long pairs, CIGAR vs oracle")
     rng = np.random.default_rng(29)
 
@@ -405,6 +405,7 @@ def test_traceback(gpu):
                                           datatype="aa"), PA, PB, O.SIM_BLOSUM62, O.DT_AA)):
         eng = I.SWAlgorithm(sw, I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
         s, ar, br = eng.compare(a, b)
+        eng.set_timing(True)
         got = eng.traceback(a, b, s, ar, br)
         mat = O.select_matrix(sim)
         for k in range(len(a)):
@@ -412,7 +413,33 @@ def test_traceback(gpu):
                           int(s[k]), mat, sw.gapInit, sw.gapExtend)
             assert np.array_equal(got[k], exp), (sw.datatype, k, I.cigar_string(got[k]),
                                                  O.cigar_string(exp))
+        # the region cells and the kernel time of the call
+        u = ar.astype(np.int64) & 0xFFFFFFFF
+        v = br.astype(np.int64) & 0xFFFFFFFF
+        cells = int(np.sum(np.where(s > 0, ((u >> 16) - (u & 0xFFFF) + 1) *
+                                    ((v >> 16) - (v & 0xFFFF) + 1), 0)))
+        ts = eng.traceback_stats()
+        assert ts["region_cells"] == cells and ts["kernel_ms"] > 0, ts
         eng.close()
+
+
+def test_traceback_int64_gaps(gpu):
+    # a gap open of 20,000,000 puts the regions past the int32 bound of the
+    # wave traceback: the one-lane int64 kernel takes the call
+    note("traceback: gap open 20,000,000 (int64 kernel), CIGAR vs oracle")
+    rng = np.random.default_rng(31)
+    A = ["".join(rng.choice(list("ACGT"), int(rng.integers(20, 200)))) for _ in range(64)]
+    B = [a[:len(a) // 2] + "T" + a[len(a) // 2:] if k % 2 else a[3:] for k, a in enumerate(A)]
+    sw = I.SWConfig(gapInit=-20_000_000, gapExtend=-1)
+    eng = I.SWAlgorithm(sw, I.IPUAlgoConfig(tilesUsed=1 << 20, maxAB=1 << 16, maxBatches=1))
+    s, ar, br = eng.compare(A, B)
+    got = eng.traceback(A, B, s, ar, br)
+    mat = O.select_matrix(O.SIM_NA)
+    for k in range(len(A)):
+        exp = O.cigar(O.encode(O.DT_NA, A[k]), O.encode(O.DT_NA, B[k]), int(ar[k]), int(br[k]),
+                      int(s[k]), mat, sw.gapInit, sw.gapExtend)
+        assert np.array_equal(got[k], exp), (k, I.cigar_string(got[k]), O.cigar_string(exp))
+    eng.close()
 
 
 def test_int16_saturation_rerun(gpu):
