@@ -1435,8 +1435,6 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
   // start first, overlap every other comparison of the call
   const int64_t kChunkDir = (int64_t)8 << 30;
   std::vector<int64_t> h_desc;
-  std::vector<uint32_t> h_cig;
-  std::vector<int32_t> h_len;
   for (int32_t c0 = 0; c0 < n;) {
     int32_t c1 = c0;
     int64_t dsum = 0, wsum = 0, csum = 0;
@@ -1499,10 +1497,13 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
     HIP_TRY(launch_traceback(T, e->stream));
     if (e->timing) HIP_TRY(hipEventRecord(e->t_ev[1], e->stream));
     if ((rc = end_work(e, e->stream))) return rc;
-    h_cig.resize((size_t)std::max<int64_t>(csum, 1));
-    h_len.resize(cn);
-    HIP_TRY(hipMemcpyAsync(h_cig.data(), T.cigar, (size_t)csum * 4, hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipMemcpyAsync(h_len.data(), T.cig_len, (size_t)cn * 4, hipMemcpyDeviceToHost, e->stream));
+    // runs and counts back through the pinned staging buffer of results (one
+    // DMA each at full link rate; the runs take each comparison's whole capacity)
+    if ((rc = ensure_pinned(e->h_meta, e->h_meta_cap, (size_t)(csum + cn) * 4 + 16))) return rc;
+    uint32_t *const h_cig = reinterpret_cast<uint32_t *>(e->h_meta);
+    int32_t *const h_len = reinterpret_cast<int32_t *>(h_cig + csum);
+    HIP_TRY(hipMemcpyAsync(h_cig, T.cigar, (size_t)csum * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(h_len, T.cig_len, (size_t)cn * 4, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->timing) {
       float ms = 0.f;
@@ -1512,7 +1513,7 @@ static int ipuma_traceback_local_body(ipuma_engine *e, int32_t n, const char *a_
     for (int32_t k = 0; k < cn; ++k) {
       const int32_t r = h_len[k];
       cigar_len[c0 + k] = r;
-      const uint32_t *src = h_cig.data() + h_desc[4 * cn + k];
+      const uint32_t *src = h_cig + h_desc[4 * cn + k];
       uint32_t *dst = cigar + cigar_offsets[c0 + k];
       for (int32_t q = 0; q < r; ++q) dst[q] = src[r - 1 - q];  // written end to start
     }
