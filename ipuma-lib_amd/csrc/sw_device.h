@@ -30,10 +30,10 @@ constexpr int64_t kBandScratchWords = 2 * 65536;
 constexpr int32_t kMaxSeqLen = 65535;
 // Rows per lane of k_trace_wave: a strip is 64 * R rows, and a lane's
 // direction bits (4 per cell) take one 32-bit word per step.  R is the fewest
-// of 2, 4, 6, 8 whose strip holds the region's |A| rows, else kTrRows.
+// (1..8) whose strip holds the region's |A| rows, else kTrRows.
 constexpr int kTrRows = 8;
 __host__ __device__ constexpr int trace_rows(int64_t m) {
-  return m <= 128 ? 2 : m <= 256 ? 4 : m <= 384 ? 6 : kTrRows;
+  return m <= 64 * kTrRows ? (int)(m > 64 ? (m + 63) / 64 : 1) : kTrRows;
 }
 // Counting-sort bins of the length bucketing: class x super-strip bin x
 // (rows of the last super-strip, m-bin).  A run is one (class, super-strip

@@ -2079,9 +2079,9 @@ __global__ __launch_bounds__(64) void k_traceback(TraceArgs T) {
 // when every value of the call's regions is bounded well inside int32: the
 // boundary value kTrNeg and its differences with real values then stay in
 // range, and every comparison decides as the int64 kernel's).  The region's
-// rows (A) run in strips of 64 * R (R = trace_rows(|A region|): 2, 4, 6 or 8
-// rows per lane, the fewest that cover a region of up to 512 rows in one
-// strip); lane k owns rows k * R .. + R - 1 of a strip and at step t computes
+// rows (A) run in strips of 64 * R (R = trace_rows(|A region|): 1 to 8 rows
+// per lane, the fewest that cover a region of up to 512 rows in one strip);
+// lane k owns rows k * R .. + R - 1 of a strip and at step t computes
 // column t - k of them,
 // receiving H and E of the row above (and the column's B code) from lane
 // k - 1 through DPP; lane 0 reads the row above the strip from the
@@ -2283,10 +2283,21 @@ __global__ __launch_bounds__(64) void k_trace_wave(TraceArgs T) {
     return;
   }
   const int m = ae - as + 1, n = be - bs + 1;
-  switch (trace_rows(m)) {
+  // The longest comparisons set the launch's end: a wave issues ahead of the
+  // other waves of its SIMD by its steps times rows (the lanes' cells)
+  const int rows = trace_rows(m);
+  const int64_t work = (int64_t)((m + 64 * rows - 1) / (64 * rows)) * (n + 63) * rows;
+  if (work > 12000) __builtin_amdgcn_s_setprio(3);
+  else if (work > 6000) __builtin_amdgcn_s_setprio(2);
+  else if (work > 3000) __builtin_amdgcn_s_setprio(1);
+  switch (rows) {
+    case 1: trace_wave<1>(T, tab, p, s, as, bs, m, n); break;
     case 2: trace_wave<2>(T, tab, p, s, as, bs, m, n); break;
+    case 3: trace_wave<3>(T, tab, p, s, as, bs, m, n); break;
     case 4: trace_wave<4>(T, tab, p, s, as, bs, m, n); break;
+    case 5: trace_wave<5>(T, tab, p, s, as, bs, m, n); break;
     case 6: trace_wave<6>(T, tab, p, s, as, bs, m, n); break;
+    case 7: trace_wave<7>(T, tab, p, s, as, bs, m, n); break;
     default: trace_wave<kTrRows>(T, tab, p, s, as, bs, m, n); break;
   }
 }

@@ -553,9 +553,9 @@ def test_traceback_emulated(emu_lib):
 
 def test_traceback_strips_emulated(emu_lib):
     """The wave traceback over regions of several strips of 512 rows (the row
-    above each strip handed over through the work area) and of one strip of 4
-    and 6 rows per lane, and the int64 kernel that gap settings beyond the
-    int32 bound take: CIGARs equal the oracle's."""
+    above each strip handed over through the work area) and of one strip of
+    3-7 rows per lane, and the int64 kernel that gap settings beyond the int32
+    bound take: CIGARs equal the oracle's."""
     import oracle as O
     I = emu_lib
     rng = np.random.default_rng(43)
@@ -576,8 +576,8 @@ def test_traceback_strips_emulated(emu_lib):
 
     dna, aa = "ACGT", "ARNDCQEGHILKMFPSTWYV"
     cases = []
-    # regions of 2-3 strips of 512 rows, and of one strip of 4 and 6 rows per lane
-    A = ["".join(rng.choice(list(dna), n)) for n in (1300, 530, 1030, 200, 350)]
+    # regions of 2-3 strips of 512 rows, and of one strip of 3-7 rows per lane
+    A = ["".join(rng.choice(list(dna), n)) for n in (1300, 530, 1030, 200, 350, 170, 420)]
     cases.append((O.SIM_NA, O.DT_NA, -2, -1, A, [mutate(a, dna) for a in A]))
     P = ["".join(rng.choice(list(aa), n)) for n in (700, 513, 300, 240)]
     cases.append((O.SIM_BLOSUM62, O.DT_AA, -10, -1, P, [mutate(a, aa) for a in P]))
