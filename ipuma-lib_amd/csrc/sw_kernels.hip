@@ -663,7 +663,7 @@ __device__ __forceinline__ int band_wait(int *prog, int need) {
 
 // Super-strip s of the band group in slot `sl`: the whole-wave pipeline body
 // of run_group, with the boundary row shared through the slot's buffer.
-template <int KP, bool PAIRKEY>
+template <int KP, bool PAIRKEY, bool TAG>
 __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__restrict__ tab, BandState &B,
                            int sl, int s, int lane, const SimdShare &share) {
   constexpr int R = kRows;
@@ -721,8 +721,9 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
       rp[q] = ((cl[2 * q] * KP + cl[2 * q + 1]) * 4u) | (((ch[2 * q] * KP + ch[2 * q + 1]) * 4u) << 16);
   }
   // keys as in run_group: one per row pair (15-bit columns) up to
-  // mpad = kPairKeyMaxM, one per row (16-bit columns) beyond
-  constexpr int KN = PAIRKEY ? R / 2 : R;
+  // mpad = kPairKeyMaxM, one per row (16-bit columns) beyond, or with TAG
+  // (every H of the strip below 4,096) row-tag keys over 16-bit columns
+  constexpr int KN = TAG ? 1 : PAIRKEY ? R / 2 : R;
   uint32_t H[R], E[R], KL[KN], KH[KN];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -789,8 +790,8 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
       col_lo = from_prev<kModeWave>(ac_lo * COLSTRIDE, col_lo, first);
       col_hi = from_prev<kModeWave>(ac_hi * COLSTRIDE, col_hi, first);
       const int j = t - k;
-      const uint32_t jinv = PAIRKEY ? ((0x7FFFu - (uint32_t)j) & 0x7FFFu) | 0x8000u
-                                    : 0xFFFFu - (uint32_t)j;
+      const uint32_t jinv = PAIRKEY && !TAG ? ((0x7FFFu - (uint32_t)j) & 0x7FFFu) | 0x8000u
+                                            : 0xFFFFu - (uint32_t)j;
       const char *tl = (const char *)tab + col_lo;
       const char *th = (const char *)tab + col_hi;
 #pragma unroll
@@ -808,7 +809,10 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
       uint32_t f = rf;
       uint32_t kl_even = 0u, kh_even = 0u;
       uint32_t jr_even = jinv, jr_odd = jinv & 0x7FFFu;
-      if constexpr (PAIRKEY) asm volatile("" : "+v"(jr_even), "+v"(jr_odd));
+      if constexpr (PAIRKEY && !TAG) asm volatile("" : "+v"(jr_even), "+v"(jr_odd));
+      u16x2 tk2[2] = {(u16x2)0, (u16x2)0};  // row-tag keys: running maxima over the rows
+      uint32_t tag_mul = 0x00100010u;       // 16 in both halves, opaque (one v_pk_mad_u16 per row)
+      if constexpr (TAG) asm volatile("" : "+v"(tag_mul));
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         s16x2 t2n = t2;
@@ -818,7 +822,12 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
         const uint32_t ho = W(__builtin_elementwise_sub_sat(U2(hn), U2(gap_o)));
         E[r] = max3h(ho, W(__builtin_elementwise_sub_sat(U2(E[r]), U2(gap_x))), kFloor2);
         f = max3h(ho, W(__builtin_elementwise_sub_sat(U2(f), U2(gap_x))), kFloor2);
-        if constexpr (PAIRKEY) {
+        if constexpr (TAG) {
+          // H * 16 + (15 - r) from the biased H, as in run_group
+          const u16x2 kr = U2(hn) * U2(tag_mul) + (unsigned short)(15 - r - 16 * kBias16);
+          if (r < 2) tk2[r] = kr;
+          else tk2[r & 1] = __builtin_elementwise_max(tk2[r & 1], kr);
+        } else if constexpr (PAIRKEY) {
           const uint32_t jr = (r & 1) ? jr_odd : jr_even;
           const uint32_t kl = (hn << 16) | jr;
           const uint32_t kh = (hn & 0xFFFF0000u) | jr;
@@ -834,6 +843,11 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
           KH[r] = max(KH[r], __builtin_amdgcn_perm(hn, jinv, 0x07060100u));
         }
         t2 = t2n;
+      }
+      if constexpr (TAG) {
+        const u16x2 tk = __builtin_elementwise_max(tk2[0], tk2[1]);
+        KL[0] = max(KL[0], __builtin_amdgcn_perm(W(tk), jinv, 0x05040100u));
+        KH[0] = max(KH[0], __builtin_amdgcn_perm(W(tk), jinv, 0x07060100u));
       }
       out_h = H[R - 1];
       out_f = f;
@@ -853,8 +867,22 @@ __device__ __forceinline__ void band_strip(const FastArgs &A, const uint32_t *__
   }
 
   int bh_lo = 0, bi_lo = 0, bj_lo = 0, bh_hi = 0, bi_hi = 0, bj_hi = 0;
+  if constexpr (TAG) {
+    const int hl = (int)(KL[0] >> 20);
+    if (hl > 0) {
+      bh_lo = hl;
+      bi_lo = row0 + 15 - (int)((KL[0] >> 16) & 15u);
+      bj_lo = 0xFFFF - (int)(KL[0] & 0xFFFFu);
+    }
+    const int hh = (int)(KH[0] >> 20);
+    if (hh > 0) {
+      bh_hi = hh;
+      bi_hi = row0 + 15 - (int)((KH[0] >> 16) & 15u);
+      bj_hi = 0xFFFF - (int)(KH[0] & 0xFFFFu);
+    }
+  }
 #pragma unroll
-  for (int q = 0; q < KN; ++q) {
+  for (int q = 0; q < (TAG ? 0 : KN); ++q) {
     const int hl = (int)(KL[q] >> 16) - kBias16;
     if (hl > bh_lo) {
       bh_lo = hl;
@@ -996,10 +1024,17 @@ __device__ __forceinline__ void band_phase(const FastArgs &A, const uint32_t *__
         // The strip ranks against the other waves on its SIMD by the work
         // left on its path (the longest path issues first, the others fill
         // its gaps), as every group of the main queue does.
-        if (__builtin_amdgcn_readfirstlane(S.gd.mpad) <= kPairKeyMaxM)
-          band_strip<KP, true>(A, tab, B, sl, nx, lane, share);
+        // Row-tag keys where no H of the strip can pass 4,095 (its rows end
+        // at (nx + 1) * 64 * kRows, and H <= smax * rows): the first strips of
+        // the small alphabet's band groups (DNA match scores are small)
+        bool tag = false;
+        if constexpr (KP == 18) tag = (int64_t)A.smax * (nx + 1) * 64 * kRows <= kTagKeyMaxH;
+        if (tag)
+          band_strip<KP, true, true>(A, tab, B, sl, nx, lane, share);
+        else if (__builtin_amdgcn_readfirstlane(S.gd.mpad) <= kPairKeyMaxM)
+          band_strip<KP, true, false>(A, tab, B, sl, nx, lane, share);
         else
-          band_strip<KP, false>(A, tab, B, sl, nx, lane, share);
+          band_strip<KP, false, false>(A, tab, B, sl, nx, lane, share);
         share_simd(share, -1, lane);
       } else {
         band_load(A, B, cur, lane, nband);
